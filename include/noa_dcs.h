@@ -1,0 +1,128 @@
+/*
+ * noa_dcs.h - C ABI of the MI355X (gfx950) PMS differential cross-section library.
+ *
+ * Evaluates the four muon DCS processes of grinisrit/noa elementwise over
+ * flat arrays of kinetic energy K (GeV) and energy transfer q (GeV) on the
+ * GPU. Results are dsigma/dnu per unit mass (m^2/kg), as in the reference.
+ *
+ * Conventions (all entry points):
+ *   - return 0 (NOA_DCS_OK) or a negative NOA_DCS_E* code; the message of the
+ *     last failure on the calling thread is noa_dcs_last_error();
+ *   - all array pointers are device pointers owned by the caller, K and q are
+ *     read, outputs are written; n is a 64-bit element count;
+ *   - work is enqueued asynchronously on `stream` (0 = the null stream);
+ *   - an output pointer that is NULL skips that process.
+ * Process ids follow PUMAS order: 0 bremsstrahlung, 1 pair production,
+ * 2 photonuclear, 3 ionisation.
+ */
+#ifndef NOA_DCS_H
+#define NOA_DCS_H
+
+#include <stdint.h>
+
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NOA_DCS_OK 0
+#define NOA_DCS_EINVAL (-1)  /* bad argument: NULL input, n < 0, bad element/mass */
+#define NOA_DCS_ELAUNCH (-2) /* a HIP call or kernel launch failed */
+#define NOA_DCS_ERANGE (-3)  /* a size exceeds a documented limit */
+#define NOA_DCS_ESTATE (-4)  /* object used in the wrong state */
+
+/* Replaces noa::pms::AtomicElement (src/noa/pms/physics.hh:39-43); same
+ * field order and layout: A in g/mol, I (mean excitation) in GeV, Z. */
+typedef struct noa_dcs_element {
+    double A;
+    double I;
+    int32_t Z;
+} noa_dcs_element;
+
+/* The four process outputs; any may be NULL. */
+typedef struct noa_dcs_outputs_f64 {
+    double* brems;
+    double* pair;
+    double* photo;
+    double* ion;
+} noa_dcs_outputs_f64;
+
+typedef struct noa_dcs_outputs_f32 {
+    float* brems;
+    float* pair;
+    float* photo;
+    float* ion;
+} noa_dcs_outputs_f32;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* noa_dcs_last_error(void);
+
+/* Library version string. */
+const char* noa_dcs_version(void);
+
+/*
+ * FP64 evaluation, bit-for-bit the reference arithmetic up to device libm.
+ * Replaces the CPU adaptors dcs::vmap(f) / dcs::pvmap(f)
+ * (src/noa/pms/dcs.hh:35-48, 62-75) for f in {bremsstrahlung,
+ * pair_production, photonuclear, ionisation} (dcs.hh:134-443) and the CUDA
+ * dcs::cuda::vmap_bremsstrahlung (src/noa/pms/dcs.cuh:30-41): one call
+ * evaluates every process whose output pointer is non-NULL.
+ */
+int noa_dcs_eval_f64(const double* K, const double* q, int64_t n, const noa_dcs_element* element,
+                     double mass, const noa_dcs_outputs_f64* out, hipStream_t stream);
+
+/*
+ * Mixed-precision evaluation: FP32 arithmetic with FP64 kept for the
+ * kinematic differences (E - q, K - q, 1 - nu), the pair-production
+ * integration bound and the photonuclear Q^2 range. FP64 inputs, FP32
+ * outputs. No reference counterpart (the reference is FP64 only); same call
+ * shape as noa_dcs_eval_f64.
+ */
+int noa_dcs_eval_f32mixed(const double* K, const double* q, int64_t n, const noa_dcs_element* element,
+                          double mass, const noa_dcs_outputs_f32* out, hipStream_t stream);
+
+/* ---- batched uneven elements: one persistent launch, captured in a hipGraph ----
+ * Element e owns pairs [offsets[e], offsets[e+1]) of the concatenated K, q
+ * and output arrays. The reference has no batched form: a caller loops
+ * dcs::vmap over elements (src/noa/pms/dcs.hh:35-48). The plan binds the
+ * buffers given at creation; noa_dcs_batch_launch replays the captured graph.
+ */
+typedef struct noa_dcs_batch noa_dcs_batch;
+
+#define NOA_DCS_PREC_F64 0
+#define NOA_DCS_PREC_F32MIXED 1
+
+typedef struct noa_dcs_batch_config {
+    int32_t precision;      /* NOA_DCS_PREC_F64 or NOA_DCS_PREC_F32MIXED */
+    int32_t tile;           /* pairs per work item; 0 = autotuned */
+    int32_t waves_per_cu;   /* persistent workgroups per CU; 0 = autotuned */
+    int32_t use_graph;      /* 1 = capture the launch in a hipGraph */
+    const char* tune_cache; /* autotune cache file, NULL = none */
+} noa_dcs_batch_config;
+
+/* elements and offsets are HOST arrays (n_elements and n_elements + 1
+ * entries); K, q and the outputs are device arrays of offsets[n_elements]
+ * pairs (outputs as double* or float* per precision, cast to void*). */
+int noa_dcs_batch_create(const noa_dcs_element* elements, const int64_t* offsets, int32_t n_elements,
+                         double mass, const double* K, const double* q, void* out_brems, void* out_pair,
+                         void* out_photo, void* out_ion, const noa_dcs_batch_config* config,
+                         hipStream_t stream, noa_dcs_batch** plan);
+int noa_dcs_batch_launch(noa_dcs_batch* plan, hipStream_t stream);
+/* Chosen tile and workgroups per CU, and the number of work items. */
+int noa_dcs_batch_info(const noa_dcs_batch* plan, int32_t* tile, int32_t* waves_per_cu, int64_t* n_items);
+int noa_dcs_batch_destroy(noa_dcs_batch* plan);
+
+/* ---- synthetic workload (counter-based, seeded) ----
+ * u(seed, stream, i) is a 53-bit uniform in [0, 1) from a SplitMix64 output
+ * of a (seed, stream, i) counter. Fills K[i] = 10^(lk0 + (lk1 - lk0) u0),
+ * nu = 10^(ln0 + (ln1 - ln0) u1), q[i] = nu K[i], with u0 = u(seed, 2s, i),
+ * u1 = u(seed, 2s + 1, i), s = stream. */
+int noa_dcs_fill_workload(double* K, double* q, int64_t n, uint64_t seed, uint64_t stream_id, double lk0,
+                          double lk1, double ln0, double ln1, hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NOA_DCS_H */

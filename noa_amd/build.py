@@ -1,0 +1,78 @@
+"""Build recipe for the gfx950 shared library (and the CPU oracle used by tests).
+
+    python -m noa_amd.build            # library + oracle (+ reference harness if present)
+
+The HIP library is compiled directly with hipcc for gfx950 only. Packed FP32
+instructions are disabled (-packed-fp32-ops); FP64 kernels switch FMA
+contraction off per function in the source.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libnoa_dcs.so")
+BUILD_DIR = os.path.join(HERE, "_build")
+
+HIP_SOURCES = ["dcs_eval.hip", "dcs_batch.hip", "surrogate.hip"]
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+HIP_FLAGS = [
+    "--offload-arch=gfx950",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-command-line-argument",
+    "-I" + os.path.join(ROOT, "include"),
+]
+
+
+def _run(cmd: list[str]) -> None:
+    res = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in res.stdout.splitlines() if "unknown target feature" not in l
+                    and "packed-fp32-ops" not in l)
+    if res.returncode != 0:
+        raise RuntimeError("build failed: " + " ".join(cmd) + "\n" + out)
+
+
+def build_library(verbose: bool = False) -> str:
+    os.makedirs(BUILD_DIR, exist_ok=True)
+    srcs = [s for s in HIP_SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    objs = [os.path.join(BUILD_DIR, os.path.splitext(s)[0] + ".o") for s in srcs]
+
+    def compile_one(i: int) -> None:
+        cmd = [HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, srcs[i]), "-o", objs[i]]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        _run(cmd)
+
+    with ThreadPoolExecutor(max_workers=len(srcs)) as ex:
+        list(ex.map(compile_one, range(len(srcs))))
+    tmp = LIB + ".tmp"
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_oracle(with_reference: bool = True) -> None:
+    oracle = os.path.join(ROOT, "oracle")
+    _run(["make", "-C", oracle, "all", f"PY={sys.executable}"])
+    if with_reference and os.path.isdir("/root/reference/src"):
+        _run(["make", "-C", oracle, "ref", f"PY={sys.executable}"])
+
+
+def main() -> None:
+    verbose = "-v" in sys.argv
+    print("building", build_library(verbose), flush=True)
+    build_oracle()
+    print("oracle built", flush=True)
+
+
+if __name__ == "__main__":
+    main()

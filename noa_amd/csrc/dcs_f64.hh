@@ -1,0 +1,244 @@
+// FP64 device physics of the four DCS processes.
+//
+// Every expression keeps the reference's evaluation order and FMA
+// contraction is off in every function body, so the only difference from the
+// reference's CPU path is device libm (ocml log/exp/log10) against glibc.
+// Launch invariants come precomputed in DcsConsts.
+//   bremsstrahlung   src/noa/pms/physics.hh:114-153
+//   pair production  src/noa/pms/dcs.hh:144-258 (8-node rule numerics.hh:110-129)
+//   photonuclear     src/noa/pms/dcs.hh:261-405 (9-node rule numerics.hh:131-152)
+//   ionisation       src/noa/pms/dcs.hh:408-443
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "dcs_consts.hh"
+
+// FMA contraction is switched off inside each function body (block scope),
+// so including this header does not change the rest of a translation unit.
+
+namespace noa_hip {
+
+// Gauss-Legendre node tables; the loop index is wave-uniform, so these are
+// scalar loads. Values: numerics.hh:116-121 (8 nodes on [0,1], truncated
+// literals kept as specified) and numerics.hh:137-144 (9 nodes on [-1,1]).
+static __constant__ double GL8_X[8] = {0.01985507, 0.10166676, 0.2372338, 0.40828268,
+                                0.59171732, 0.7627662, 0.89833324, 0.98014493};
+static __constant__ double GL8_W[8] = {0.05061427, 0.11119052, 0.15685332, 0.18134189,
+                                0.18134189, 0.15685332, 0.11119052, 0.05061427};
+static __constant__ double GL9_X[9] = {0.0000000000000000, -0.8360311073266358, 0.8360311073266358,
+                                -0.9681602395076261, 0.9681602395076261, -0.3242534234038089,
+                                0.3242534234038089, -0.6133714327005904, 0.6133714327005904};
+static __constant__ double GL9_W[9] = {0.3302393550012598, 0.1806481606948574, 0.1806481606948574,
+                                0.0812743883615744, 0.0812743883615744, 0.3123470770400029,
+                                0.3123470770400029, 0.2606106964029354, 0.2606106964029354};
+
+struct F64 {
+    using out_t = double;
+    static constexpr bool has_redo = false;
+
+    // physics.hh:114-153
+    static __device__ __forceinline__ double brems(double K, double q, const DcsConsts& c) {
+#pragma clang fp contract(off)
+        const double me = ELECTRON_MASS;
+        const double sqrte = SQRTE_BREMS;
+        const double E = K + c.mass;
+        const double delta_factor = c.b_half_m2 / E;
+        const double qe_max = E / (1. + c.b_half_m2 / (me * E));
+        const double nu = q / E;
+        const double delta = delta_factor * nu / (1. - nu);
+        double Phi_n = log(c.b_BZ_n * (c.mass + delta * c.b_Dn_sqrte_m2) /
+                           (c.b_D_n * (me + delta * sqrte * c.b_BZ_n)));
+        if (Phi_n < 0.) Phi_n = 0.;
+        double Phi_e = 0.;
+        if (q < qe_max) {
+            Phi_e = log(c.b_BZe_mass / ((1. + delta * c.b_phie_factor) * (me + delta * sqrte * c.b_BZ_e)));
+            if (Phi_e < 0.) Phi_e = 0.;
+        }
+        const double dcs = c.b_dcs_factor * (c.Zd * Phi_n + Phi_e) * (4. / 3. * (1. / nu - 1.) + nu);
+        return (dcs < 0.) ? 0. : dcs * 1E+03 * AVOGADRO_NUMBER / c.A;
+    }
+
+    // dcs.hh:408-443
+    static __device__ __forceinline__ double ion(double K, double q, const DcsConsts& c) {
+#pragma clang fp contract(off)
+        const double me = ELECTRON_MASS;
+        const double P2 = K * (K + c.i_2m);
+        const double E = K + c.mass;
+        const double Wmax = 2. * me * P2 / (c.i_m2 + me * (me + 2. * E));
+        if ((Wmax < X_FRACTION * K) || (q > Wmax)) return 0.;
+        if (q <= c.i_Wmin) return 0.;
+        const double a0 = 0.5 / P2;
+        const double a1 = -1. / Wmax;
+        const double a2 = E * E / P2;
+        const double cs = 1.535336E-05 * E * c.Zd / c.A * (a0 + 1. / q * (a1 + a2 / q));
+        double Delta = 0.;
+        if (K >= c.i_Kthr) {
+            const double L1 = log(1. + 2. * q / me);
+            Delta = 1.16141E-03 * L1 * (log(4. * E * (E - q) / c.i_m2) - L1);
+        }
+        return cs * (1. + Delta);
+    }
+
+    // dcs.hh:151-156 and 168-175: the exits before the integral.
+    static __device__ __forceinline__ bool pair_active(double K, double q, const DcsConsts& c) {
+#pragma clang fp contract(off)
+        if (q <= 4. * ELECTRON_MASS) return false;
+        if (q >= K + c.p_cut) return false;
+        const double gamma = 1. + K / c.mass;
+        const double x0 = 4. * ELECTRON_MASS / q;
+        const double x1 = 6. / (gamma * (gamma - q / c.mass));
+        const double argmin = (x0 + 2. * (1. - x0) * x1) / (1. + (1. - x1) * sqrt(1. - x0));
+        return !((argmin >= 1.) || (argmin <= 0.));
+    }
+
+    // dcs.hh:144-258 for a pair that passed pair_active.
+    static __device__ double pair(double K, double q, const DcsConsts& c) {
+#pragma clang fp contract(off)
+        const double me = ELECTRON_MASS;
+        const double nu = q / (K + c.mass);
+        const double beta = 0.5 * nu * nu / (1. - nu);
+        const double xi_factor = c.p_hr2 * beta;
+        const double gamma = 1. + K / c.mass;
+        const double x0 = 4. * me / q;
+        const double x1 = 6. / (gamma * (gamma - q / c.mass));
+        const double argmin = (x0 + 2. * (1. - x0) * x1) / (1. + (1. - x1) * sqrt(1. - x0));
+        const double tmin = log(argmin);
+
+        double I = 0.;
+#pragma unroll 1
+        for (int j = 0; j < 8; j++) {
+            const double t = GL8_X[j];
+            const double eps = exp(t * tmin);
+            const double rho = 1. - eps;
+            const double rho2 = rho * rho;
+            const double rho21 = eps * (2. - eps);
+            const double xi = xi_factor * rho21;
+            const double xi_i = 1. / xi;
+            double Be;
+            if (xi >= 1E+03)
+                Be = 0.5 * xi_i * ((3. - rho2) + 2. * beta * (1. + rho2));
+            else
+                Be = ((2. + rho2) * (1. + beta) + xi * (3. + rho2)) * log(1. + xi_i) +
+                     (rho21 - beta) / (1. + xi) - 3. - rho2;
+            const double Ye = (5. - rho2 + 4. * beta * (1. + rho2)) /
+                              (2. * (1. + 3. * beta) * log(3. + xi_i) - rho2 - 2. * beta * (2. - rho2));
+            const double xe = (1. + xi) * (1. + Ye);
+            const double cLi = c.p_cL / rho21;
+            const double Le = log(c.p_AZ13 * sqrt(xe) * q / (q + cLi * xe)) - 0.5 * log(1. + c.p_cLe * xe);
+            double Phi_e = Be * Le;
+            if (Phi_e < 0.) Phi_e = 0.;
+            double Bmu;
+            if (xi <= 1E-03)
+                Bmu = 0.5 * xi * (5. - rho2 + beta * (3. + rho2));
+            else
+                Bmu = ((1. + rho2) * (1. + 1.5 * beta) - xi_i * (1. + 2. * beta) * rho21) * log(1. + xi) +
+                      xi * (rho21 - beta) / (1. + xi) + (1. + 2. * beta) * rho21;
+            const double Ymu = (4. + rho2 + 3. * beta * (1. + rho2)) /
+                               ((1. + rho2) * (1.5 + 2. * beta) * log(3. + xi) + 1. - 1.5 * rho2);
+            const double xmu = (1. + xi) * (1. + Ymu);
+            const double Lmu = log(c.p_rAZ13 * q / (c.p_15Z13 * (q + cLi * xmu)));
+            double Phi_mu = Bmu * Lmu;
+            if (Phi_mu < 0.) Phi_mu = 0.;
+            const double f = -(Phi_e + Phi_mu / c.p_r2) * (1. - rho) * tmin;
+            I += f * 1. * GL8_W[j];
+        }
+
+        double zeta;
+        if (gamma <= 35.) {
+            zeta = 0.;
+        } else {
+            zeta = 0.073 * log(gamma / (1. + c.p_g1 * gamma * c.p_Z13 * c.p_Z13)) - 0.26;
+            if (zeta <= 0.)
+                zeta = 0.;
+            else
+                zeta /= 0.058 * log(gamma / (1. + c.p_g2 * gamma * c.p_Z13)) - 0.14;
+        }
+        const double E = K + c.mass;
+        const double dcs = c.p_dcs_c * (c.Zd + zeta) * (E - q) * I / (q * E);
+        return (dcs < 0.) ? 0. : dcs * 1E+03 * AVOGADRO_NUMBER * (c.mass + K) / c.A;
+    }
+
+    // dcs.hh:357-381: the exits before the integral.
+    static __device__ __forceinline__ bool photo_active(double K, double q, const DcsConsts& c) {
+#pragma clang fp contract(off)
+        if ((q < 1.) || (q < 2E-03 * K)) return false;
+        const double E = K + c.mass;
+        if ((q >= (E - c.mass)) || (q <= c.ph_thr_pi)) return false;
+        const double y = q / E;
+        const double Q2min = c.ph_m2 * y * y / (1. - y);
+        const double Q2max = 2.0 * PHOTO_M * (q - PHOTO_MPI) - PHOTO_MPI * PHOTO_MPI;
+        return !((Q2max < Q2min) | (Q2min < 0.));
+    }
+
+    // dcs.hh:261-405 for a pair that passed photo_active.
+    static __device__ double photo(double K, double q, const DcsConsts& c) {
+#pragma clang fp contract(off)
+        const double M = PHOTO_M;
+        const double mpi = PHOTO_MPI;
+        const double E = K + c.mass;
+        const double y = q / E;
+        const double Q2min = c.ph_m2 * y * y / (1. - y);
+        const double Q2max = 2.0 * M * (q - mpi) - mpi * mpi;
+        const double pQ2min = log(Q2min);
+        const double pQ2max = log(Q2max);
+        const double dpQ2 = pQ2max - pQ2min;
+        const double pQ2c = 0.5 * (pQ2max + pQ2min);
+        const double hd = 0.5 * dpQ2;
+        const double Mq = M * q;
+        const double EE = E * E;
+
+        double ds = 0.;
+#pragma unroll 1
+        for (int j = 0; j < 9; j++) {
+            const double Q2 = exp(pQ2c + hd * GL9_X[j]);
+            // dcs_photonuclear_d2 (dcs.hh:335-355)
+            const double x = 0.5 * Q2 / Mq;
+            // dcs_photonuclear_f2_allm (dcs.hh:261-307)
+            const double M2 = 0.8803505929;
+            const double W2 = M2 + Q2 * (1.0 / x - 1.0);
+            const double t = log(log((Q2 + ALLM_Q02) / ALLM_LAMBDA2) / c.ph_L0);
+            const double xP = (Q2 + 49.457) / (Q2 + 49.457 + W2 - M2);
+            const double xR = (Q2 + 0.15052) / (Q2 + 0.15052 + W2 - M2);
+            const double lnt = log(t);
+            const double cP = 0.28067 + (0.28067 - 0.22291) * (1.0 / (1.0 + exp(2.1979 * lnt)) - 1.0);
+            const double aP = -0.0808 + (-0.0808 - -0.44812) * (1.0 / (1.0 + exp(1.1709 * lnt)) - 1.0);
+            const double bP = 0.36292 + 1.8917 * exp(1.8439 * lnt);
+            const double cR = 0.80107 + 0.97307 * exp(3.4942 * lnt);
+            const double aR = 0.58400 + 0.37888 * exp(2.6063 * lnt);
+            const double bR = 0.01147 + 3.7582 * exp(0.49338 * lnt);
+            const double l1x = log(1. - x);
+            const double F2P = cP * exp(aP * log(xP) + bP * l1x);
+            const double F2R = cR * exp(aR * log(xR) + bR * l1x);
+            const double F2p = Q2 / (Q2 + 0.31985) * (F2P + F2R);
+            // dcs_photonuclear_f2a_drss (dcs.hh:310-319)
+            double a = 1.0;
+            if (x < 0.0014)
+                a = c.ph_a_low;
+            else if (x < 0.04)
+                a = exp((0.069 * log10(x) + 0.097) * c.ph_logA);
+            const double F2A = c.ph_halfA * a * (2.0 + x * (-1.85 + x * (2.45 + x * (-2.35 + x)))) * F2p;
+            // dcs_photonuclear_r_whitlow (dcs.hh:322-332)
+            double q2 = Q2;
+            if (Q2 < 0.3) q2 = 0.3;
+            const double theta = 1. + 12.0 * q2 / (1.0 + q2) * 0.015625 / (0.015625 + x * x);
+            const double R = 0.635 / log(q2 / 0.04) * theta + 0.5747 / q2 - 0.3534 / (0.09 + q2 * q2);
+            const double dds =
+                (1. - y + 0.5 * (1. - c.ph_2m2 / Q2) * (y * y + Q2 / EE) / (1. + R)) / (Q2 * Q2) -
+                0.25 / (EE * Q2);
+            const double d2 = 2.603096E-35 * F2A * dds / q;
+            ds += d2 * Q2 * 1. * GL9_W[j];
+        }
+        return (ds < 0.) ? 0. : 0.5 * ds * dpQ2 * 1E+03 * AVOGADRO_NUMBER * (c.mass + K) / c.A;
+    }
+
+    // The tile runner's interface; FP64 results are final (no recomputation).
+    static __device__ __forceinline__ double pair(double K, double q, const DcsConsts& c, bool&) {
+        return pair(K, q, c);
+    }
+    static __device__ __forceinline__ double photo(double K, double q, const DcsConsts& c, bool&) {
+        return photo(K, q, c);
+    }
+};
+
+}  // namespace noa_hip

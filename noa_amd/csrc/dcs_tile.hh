@@ -1,0 +1,155 @@
+// One tile of (K, q) pairs through all four DCS processes, with in-LDS
+// compaction of the two quadrature processes.
+//
+// Pair production is active on ~59% and photonuclear on ~23% of uniformly
+// drawn pairs, so a lane-per-pair map leaves most of a wave idle inside the
+// 8- and 9-node integrals. A tile is therefore evaluated in phases:
+//   1. each lane takes TILE/WG pairs with coalesced loads, computes the two
+//      closed-form processes (bremsstrahlung, ionisation), runs the cheap
+//      kinematic exits of the two integral processes, writes 0 where they
+//      exit, and appends the surviving pair to a per-process list in LDS
+//      (wave ballot + one LDS atomic per wave);
+//   2. all lanes drain the two lists densely, heaviest process first, so
+//      every wave runs the integrals with (almost) all lanes active;
+//   3. (mixed precision only) pairs whose FP32 integral failed its own error
+//      estimate were appended to a third list in phase 2 and are recomputed
+//      densely with the FP64 physics.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "dcs_consts.hh"
+#include "dcs_f64.hh"
+
+namespace noa_hip {
+
+constexpr int WG = 256;  // 4 wave64s
+
+template <class OutT>
+struct Outs {
+    OutT* b;
+    OutT* p;
+    OutT* ph;
+    OutT* i;
+};
+
+template <int TILE>
+struct TileLists {
+    static_assert(TILE <= 32768, "15-bit tile offsets (bit 15 tags the process in the redo list)");
+    uint16_t photo[TILE];
+    uint16_t pair[TILE];
+    uint16_t redo[TILE];
+    uint32_t n_photo;
+    uint32_t n_pair;
+    uint32_t n_redo;
+};
+
+constexpr uint32_t REDO_PHOTO = 0x8000u;
+
+// Appends `value` for every active lane with pred set; one LDS atomic per
+// wave. Lanes outside the current EXEC mask take no part, so it may be called
+// from divergent code as long as every lane with pred set is active.
+__device__ __forceinline__ void wave_append(bool pred, uint32_t* counter, uint16_t* list, uint32_t value) {
+    const unsigned long long m = __ballot(pred);
+    if (m == 0ull) return;
+    const uint32_t rank = __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
+    uint32_t base = 0;
+    if (pred && rank == 0) base = atomicAdd(counter, uint32_t(__popcll(m)));
+    const int leader = __ffsll((long long)m) - 1;
+    base = uint32_t(__builtin_amdgcn_readlane(int(base), leader));
+    if (pred) list[base + rank] = uint16_t(value);
+}
+
+// Evaluates pairs [base, end) (end - base <= TILE) of one element. Must be
+// called by all WG threads of the workgroup.
+template <class P, int TILE>
+__device__ __forceinline__ void run_tile(int64_t base, int64_t end, const double* __restrict__ K,
+                                         const double* __restrict__ q, const DcsConsts& c,
+                                         const Outs<typename P::out_t>& o, TileLists<TILE>& s) {
+    using out_t = typename P::out_t;
+    constexpr int ITEMS = TILE / WG;
+    const int tid = threadIdx.x;
+    if (tid == 0) {
+        s.n_photo = 0;
+        s.n_pair = 0;
+        s.n_redo = 0;
+    }
+    __syncthreads();
+
+    // phase 1: closed forms, exits, compaction
+    const bool want_pair = o.p != nullptr;
+    const bool want_photo = o.ph != nullptr;
+    int64_t gi = base + tid;
+    double k_next = 0., q_next = 0.;
+    if (gi < end) {
+        k_next = K[gi];
+        q_next = q[gi];
+    }
+#pragma unroll 1
+    for (int it = 0; it < ITEMS; it++) {
+        const int local = it * WG + tid;
+        gi = base + local;
+        const bool valid = gi < end;
+        const double k = k_next, qq = q_next;
+        if (it + 1 < ITEMS && gi + WG < end) {  // prefetch the next pair of this lane
+            k_next = K[gi + WG];
+            q_next = q[gi + WG];
+        }
+        bool pa = false, pha = false;
+        if (valid) {
+            if (o.b) o.b[gi] = P::brems(k, qq, c);
+            if (o.i) o.i[gi] = P::ion(k, qq, c);
+            if (want_pair) {
+                pa = P::pair_active(k, qq, c);
+                if (!pa) o.p[gi] = out_t(0);
+            }
+            if (want_photo) {
+                pha = P::photo_active(k, qq, c);
+                if (!pha) o.ph[gi] = out_t(0);
+            }
+        }
+        if (want_photo) wave_append(pha, &s.n_photo, s.photo, uint32_t(local));
+        if (want_pair) wave_append(pa, &s.n_pair, s.pair, uint32_t(local));
+    }
+    __syncthreads();
+
+    // phase 2: dense integrals, photonuclear (heavier) first
+    const uint32_t nph = s.n_photo;
+    const uint32_t total = nph + s.n_pair;
+#pragma unroll 1
+    for (uint32_t v = tid; v < total; v += WG) {
+        bool redo = false;
+        uint32_t tag;
+        if (v < nph) {
+            tag = s.photo[v];
+            const int64_t g = base + tag;
+            o.ph[g] = P::photo(K[g], q[g], c, redo);
+            tag |= REDO_PHOTO;
+        } else {
+            tag = s.pair[v - nph];
+            const int64_t g = base + tag;
+            o.p[g] = P::pair(K[g], q[g], c, redo);
+        }
+        if (P::has_redo) wave_append(redo, &s.n_redo, s.redo, tag);
+    }
+
+    // phase 3: FP64 recomputation of the pairs that failed their estimate
+    if (P::has_redo) {
+        __syncthreads();
+        const uint32_t nr = s.n_redo;
+#pragma unroll 1
+        for (uint32_t v = tid; v < nr; v += WG) {
+            const uint32_t tag = s.redo[v];
+            const int64_t g = base + (tag & (REDO_PHOTO - 1));
+            if (tag & REDO_PHOTO)
+                o.ph[g] = out_t(F64::photo(K[g], q[g], c));
+            else
+                o.p[g] = out_t(F64::pair(K[g], q[g], c));
+        }
+    }
+    __syncthreads();  // the lists are reused by the next tile of a persistent workgroup
+}
+
+}  // namespace noa_hip

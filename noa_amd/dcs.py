@@ -1,0 +1,203 @@
+"""The `noa::pms::dcs` call surface on MI355X.
+
+Mirrors the reference's tensor adaptors (src/noa/pms/dcs.hh:35-87) and its
+GPU entry points (src/noa/pms/dcs.hh:1004-1019, dcs.cuh:30-51):
+
+    result = torch.zeros_like(K)
+    dcs.vmap(dcs.pair_production)(result, K, q, STANDARD_ROCK, MUON_MASS)
+    brems = dcs.map(dcs.bremsstrahlung)(K, q, STANDARD_ROCK, MUON_MASS)
+    dcs.hip.vmap_bremsstrahlung(result, K, q, STANDARD_ROCK, MUON_MASS)
+    b, p, ph, i = dcs.hip.map_all(K, q, STANDARD_ROCK, MUON_MASS)
+
+Tensors must be float64, contiguous, on a ROCm (torch "cuda") device, with
+K, q and result of equal numel. Unlike the reference (which checks only the
+dtype, through data_ptr<double>()), size, device and contiguity are checked
+and raise ValueError. Work is enqueued on torch's current stream. There is
+no CPU path: `pvmap`/`pmap` (the reference's OpenMP forms) run the same GPU
+kernels as `vmap`/`map`.
+"""
+from __future__ import annotations
+
+import ctypes
+from types import SimpleNamespace
+
+import torch
+
+from . import _lib
+from .physics import AtomicElement, MUON_MASS, STANDARD_ROCK  # noqa: F401  (re-exported)
+
+__all__ = [
+    "Process", "bremsstrahlung", "pair_production", "photonuclear", "ionisation", "PROCESSES",
+    "vmap", "map", "pvmap", "pmap", "hip", "relative_error", "mean_error",
+]
+
+
+class Process:
+    """One DCS process; ids follow PUMAS order (0 brems, 1 pair, 2 photo, 3 ion)."""
+
+    def __init__(self, pid: int, name: str, ref: str):
+        self.id = pid
+        self.name = name
+        self.ref = ref
+
+    def __repr__(self) -> str:
+        return f"dcs.{self.name}"
+
+    def __call__(self, kinetic_energy: float, recoil_energy: float, element: AtomicElement,
+                 mass: float) -> float:
+        """Scalar form of the reference functor (one pair evaluated on the GPU)."""
+        dev = torch.device("cuda", torch.cuda.current_device())
+        k = torch.tensor([float(kinetic_energy)], dtype=torch.float64, device=dev)
+        q = torch.tensor([float(recoil_energy)], dtype=torch.float64, device=dev)
+        return float(map(self)(k, q, element, mass).item())
+
+
+bremsstrahlung = Process(0, "bremsstrahlung", "src/noa/pms/physics.hh:114-153, dcs.hh:134-140")
+pair_production = Process(1, "pair_production", "src/noa/pms/dcs.hh:144-258")
+photonuclear = Process(2, "photonuclear", "src/noa/pms/dcs.hh:362-405")
+ionisation = Process(3, "ionisation", "src/noa/pms/dcs.hh:408-443")
+PROCESSES = (bremsstrahlung, pair_production, photonuclear, ionisation)
+
+
+def _element(el) -> _lib.Element:
+    return _lib.Element(float(el.A), float(el.I), int(el.Z))
+
+
+def _check_inputs(K: torch.Tensor, q: torch.Tensor) -> None:
+    for name, t in (("kinetic_energies", K), ("recoil_energies", q)):
+        if not isinstance(t, torch.Tensor):
+            raise TypeError(f"{name} must be a torch.Tensor")
+        if t.dtype != torch.float64:
+            raise ValueError(f"{name} must be float64 (got {t.dtype})")
+        if t.device.type != "cuda":
+            raise ValueError(f"{name} must be on a GPU device (got {t.device}); there is no CPU path")
+        if not t.is_contiguous():
+            raise ValueError(f"{name} must be contiguous")
+    if q.numel() != K.numel():
+        raise ValueError(f"recoil_energies has {q.numel()} elements, kinetic_energies {K.numel()}")
+    if q.device != K.device:
+        raise ValueError("kinetic_energies and recoil_energies must be on the same device")
+
+
+def _check_output(r: torch.Tensor | None, K: torch.Tensor, dtype: torch.dtype) -> None:
+    if r is None:
+        return
+    if not isinstance(r, torch.Tensor):
+        raise TypeError("result must be a torch.Tensor")
+    if r.dtype != dtype:
+        raise ValueError(f"result must be {dtype} (got {r.dtype})")
+    if r.device != K.device:
+        raise ValueError(f"result is on {r.device}, inputs on {K.device}")
+    if not r.is_contiguous():
+        raise ValueError("result must be contiguous")
+    if r.numel() != K.numel():
+        raise ValueError(f"result has {r.numel()} elements, kinetic_energies {K.numel()}")
+
+
+def _ptr(t: torch.Tensor | None):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def _stream(K: torch.Tensor):
+    return ctypes.c_void_p(torch.cuda.current_stream(K.device).cuda_stream)
+
+
+def eval_f64(K: torch.Tensor, q: torch.Tensor, element: AtomicElement, mass: float,
+             brems=None, pair=None, photo=None, ion=None) -> None:
+    """One fused launch writing every non-None output (FP64)."""
+    _check_inputs(K, q)
+    for r in (brems, pair, photo, ion):
+        _check_output(r, K, torch.float64)
+    out = _lib.OutputsF64(*(_ptr(r) for r in (brems, pair, photo, ion)))
+    el = _element(element)
+    with torch.cuda.device(K.device):
+        _lib.check(_lib.lib().noa_dcs_eval_f64(_ptr(K), _ptr(q), K.numel(), ctypes.byref(el), float(mass),
+                                               ctypes.byref(out), _stream(K)))
+
+
+def eval_f32mixed(K: torch.Tensor, q: torch.Tensor, element: AtomicElement, mass: float,
+                  brems=None, pair=None, photo=None, ion=None) -> None:
+    """Mixed-precision form: float64 inputs, float32 outputs."""
+    _check_inputs(K, q)
+    for r in (brems, pair, photo, ion):
+        _check_output(r, K, torch.float32)
+    out = _lib.OutputsF32(*(_ptr(r) for r in (brems, pair, photo, ion)))
+    el = _element(element)
+    with torch.cuda.device(K.device):
+        _lib.check(_lib.lib().noa_dcs_eval_f32mixed(_ptr(K), _ptr(q), K.numel(), ctypes.byref(el), float(mass),
+                                                    ctypes.byref(out), _stream(K)))
+
+
+def _process(f) -> Process:
+    if not isinstance(f, Process):
+        raise TypeError(f"expected one of {PROCESSES}, got {f!r}")
+    return f
+
+
+def vmap(dcs_func: Process):
+    """dcs.hh:35-48: returns f(result, K, q, element, mass) writing in place."""
+    p = _process(dcs_func)
+
+    def run(result: torch.Tensor, kinetic_energies: torch.Tensor, recoil_energies: torch.Tensor,
+            element: AtomicElement, mass: float) -> None:
+        outs = [None] * 4
+        outs[p.id] = result
+        _check_output(result, kinetic_energies, torch.float64)
+        eval_f64(kinetic_energies, recoil_energies, element, mass, *outs)
+    return run
+
+
+def map(dcs_func: Process):  # noqa: A001  (the reference's name)
+    """dcs.hh:50-60: returns f(K, q, element, mass) -> new tensor like K."""
+    p = _process(dcs_func)
+
+    def run(kinetic_energies: torch.Tensor, recoil_energies: torch.Tensor, element: AtomicElement,
+            mass: float) -> torch.Tensor:
+        result = torch.zeros_like(kinetic_energies)
+        vmap(p)(result, kinetic_energies, recoil_energies, element, mass)
+        return result
+    return run
+
+
+pvmap = vmap   # dcs.hh:62-75 (OpenMP form); on the GPU the same kernels
+pmap = map     # dcs.hh:77-87
+
+
+def _hip_namespace() -> SimpleNamespace:
+    ns = {}
+    for p in PROCESSES:
+        ns[f"vmap_{p.name}"] = vmap(p)
+        ns[f"map_{p.name}"] = map(p)
+
+    def vmap_all(results, kinetic_energies, recoil_energies, element, mass) -> None:
+        """Fused 4-process launch into results = (brems, pair, photo, ion)."""
+        eval_f64(kinetic_energies, recoil_energies, element, mass, *results)
+
+    def map_all(kinetic_energies, recoil_energies, element, mass):
+        res = tuple(torch.zeros_like(kinetic_energies) for _ in range(4))
+        vmap_all(res, kinetic_energies, recoil_energies, element, mass)
+        return res
+
+    def map_all_f32mixed(kinetic_energies, recoil_energies, element, mass):
+        res = tuple(torch.zeros(kinetic_energies.shape, dtype=torch.float32, device=kinetic_energies.device)
+                    for _ in range(4))
+        eval_f32mixed(kinetic_energies, recoil_energies, element, mass, *res)
+        return res
+
+    ns.update(vmap_all=vmap_all, map_all=map_all, map_all_f32mixed=map_all_f32mixed)
+    return SimpleNamespace(**ns)
+
+
+hip = _hip_namespace()
+cuda = hip  # the reference's dcs::cuda::{vmap,map}_bremsstrahlung names (dcs.hh:1004-1019)
+
+
+def relative_error(computed: torch.Tensor, expected: torch.Tensor) -> torch.Tensor:
+    """common.hh:228-236: mean(|c - e| / (c + numeric_limits<T>::min()))."""
+    tiny = torch.finfo(computed.dtype).tiny
+    return torch.abs((computed - expected) / (computed + tiny)).mean()
+
+
+def mean_error(computed: torch.Tensor, expected: torch.Tensor) -> torch.Tensor:
+    """common.hh:238-240."""
+    return torch.abs(computed - expected).mean()

@@ -1,0 +1,104 @@
+// C entry points of the CPU oracle (test infrastructure only: loaded by
+// tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg).
+//
+// oracle_dcs_map mirrors the reference's element loops
+// (src/noa/utils/common.hh:146-183): the serial for_eachi when threads == 1,
+// and pfor_eachi's `#pragma omp parallel for` with the default static
+// schedule otherwise. The reference maps one process per call
+// (src/noa/pms/dcs.hh:35-87); process ids follow PUMAS order:
+// 0 bremsstrahlung, 1 pair production, 2 photonuclear, 3 ionisation.
+#include <cstdint>
+#include <cstring>
+
+#include <omp.h>
+
+#include "dcs_oracle.hh"
+#include "opcount.hh"
+
+namespace noa_oracle {
+thread_local OpTally g_tally;
+}
+
+using noa_oracle::Counted;
+using noa_oracle::Element;
+
+namespace {
+
+template <class T>
+T eval_process(int process, T K, T q, const Element<T>& el, T mass) {
+    switch (process) {
+        case 0: return noa_oracle::bremsstrahlung<T>(K, q, el, mass);
+        case 1: return noa_oracle::pair_production<T>(K, q, el, mass);
+        case 2: return noa_oracle::photonuclear<T>(K, q, el, mass);
+        default: return noa_oracle::ionisation<T>(K, q, el, mass);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+// result[i] = dcs_process(K[i], q[i], element, mass) for i < n.
+// Returns 0, or -1 on a bad process id.
+int oracle_dcs_map(int process, const double* K, const double* q, int64_t n, double A, double I,
+                   int32_t Z, double mass, double* result, int threads) {
+    if (process < 0 || process > 3) return -1;
+    const Element<double> el{A, I, Z};
+    if (threads == 1) {
+        for (int64_t i = 0; i < n; i++) result[i] = eval_process<double>(process, K[i], q[i], el, mass);
+    } else {
+        if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for default(none) shared(n, K, q, el, mass, result, process)
+        for (int64_t i = 0; i < n; i++) result[i] = eval_process<double>(process, K[i], q[i], el, mass);
+    }
+    return 0;
+}
+
+// Four processes in sequence (one map per process, as a reference caller
+// would do it); out is process-major: out[p * n + i].
+int oracle_dcs_map_all(const double* K, const double* q, int64_t n, double A, double I, int32_t Z,
+                       double mass, double* out, int threads) {
+    for (int p = 0; p < 4; p++) {
+        const int rc = oracle_dcs_map(p, K, q, n, A, I, Z, mass, out + p * n, threads);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+// The Gauss-Legendre rule alone (numerics.hh:72-89) over f(x) = x^k on
+// [lb, ub], so that tests can pin the node tables with closed forms.
+double oracle_quadrature_monomial(int rule, int k, double lb, double ub, uint32_t min_points) {
+    auto f = [k](double x) {
+        double r = 1.0;
+        for (int i = 0; i < k; i++) r *= x;
+        return r;
+    };
+    switch (rule) {
+        case 6:
+            return noa_oracle::gauss_legendre<double>(lb, ub, f, min_points, 6, noa_oracle::GL6_X,
+                                                      noa_oracle::GL6_W);
+        case 8:
+            return noa_oracle::gauss_legendre<double>(lb, ub, f, min_points, 8, noa_oracle::GL8_X,
+                                                      noa_oracle::GL8_W);
+        default:
+            return noa_oracle::gauss_legendre<double>(lb, ub, f, min_points, 9, noa_oracle::GL9_X,
+                                                      noa_oracle::GL9_W);
+    }
+}
+
+// Op tally of one process over n pairs, summed: counts[k] for k < 9 in the
+// order add, mul, div, sqrt, log, exp, log10, pow, cmp.
+int oracle_opcount(int process, const double* K, const double* q, int64_t n, double A, double I,
+                   int32_t Z, double mass, uint64_t* counts) {
+    if (process < 0 || process > 3) return -1;
+    std::memset(&noa_oracle::g_tally, 0, sizeof(noa_oracle::g_tally));
+    const Element<Counted> el{Counted(A), Counted(I), Z};
+    for (int64_t i = 0; i < n; i++)
+        eval_process<Counted>(process, Counted(K[i], true), Counted(q[i], true), el, Counted(mass));
+    for (int k = 0; k < noa_oracle::OP_NKINDS; k++) counts[k] = noa_oracle::g_tally.n[k];
+    return 0;
+}
+
+int oracle_max_threads(void) { return omp_get_max_threads(); }
+
+}  // extern "C"

@@ -1,0 +1,72 @@
+// Build harness for the reference implementation itself (test infrastructure
+// only). It is compiled from the reference's own headers where they lie
+// (grinisrit/noa src/noa/pms/dcs.hh, src/noa/pms/physics.hh,
+// src/noa/utils/common.hh, src/noa/utils/numerics.hh) against the LibTorch
+// that ships with this image, into oracle/_ref/. It contains no reference
+// code: it only calls the reference's public adaptors through a C ABI so the
+// CPU oracle and the golden fixtures can be checked against the real thing.
+#include <cstdint>
+
+#include <noa/pms/dcs.hh>
+#include <noa/pms/physics.hh>
+#include <noa/utils/common.hh>
+#include <noa/utils/numerics.hh>
+
+namespace dcs = noa::pms::dcs;
+
+namespace {
+
+torch::Tensor wrap(const double* p, int64_t n) {
+    return torch::from_blob(const_cast<double*>(p), {n}, torch::TensorOptions().dtype(torch::kFloat64));
+}
+
+template <class F>
+void run(const F& f, bool parallel, const torch::Tensor& r, const torch::Tensor& k, const torch::Tensor& q,
+         const noa::pms::AtomicElement& el, double mass) {
+    if (parallel)
+        dcs::pvmap(f)(r, k, q, el, mass);  // dcs.hh:62-75
+    else
+        dcs::vmap(f)(r, k, q, el, mass);  // dcs.hh:35-48
+}
+
+}  // namespace
+
+extern "C" {
+
+// result[i] = reference functor(K[i], q[i], element, mass). Process ids:
+// 0 bremsstrahlung, 1 pair production, 2 photonuclear, 3 ionisation.
+int ref_dcs_map(int process, const double* K, const double* q, int64_t n, double A, double I, int32_t Z,
+                double mass, double* result, int parallel) {
+    const auto k = wrap(K, n), qq = wrap(q, n), r = wrap(result, n);
+    const noa::pms::AtomicElement el{A, I, Z};
+    switch (process) {
+        case 0: run(dcs::bremsstrahlung, parallel, r, k, qq, el, mass); break;
+        case 1: run(dcs::pair_production, parallel, r, k, qq, el, mass); break;
+        case 2: run(dcs::photonuclear, parallel, r, k, qq, el, mass); break;
+        case 3: run(dcs::ionisation, parallel, r, k, qq, el, mass); break;
+        default: return -1;
+    }
+    return 0;
+}
+
+// noa::utils::relative_error (common.hh:228-236).
+double ref_relative_error(const double* computed, const double* expected, int64_t n) {
+    return noa::utils::relative_error(wrap(computed, n), wrap(expected, n)).item<double>();
+}
+
+// noa::utils::numerics::quadrature{6,8,9} (numerics.hh:91-152) over x^k.
+double ref_quadrature_monomial(int rule, int k, double lb, double ub, uint32_t min_points) {
+    auto f = [k](const double& x) {
+        double r = 1.0;
+        for (int i = 0; i < k; i++) r *= x;
+        return r;
+    };
+    namespace num = noa::utils::numerics;
+    switch (rule) {
+        case 6: return num::quadrature6<double>(lb, ub, f, min_points);
+        case 8: return num::quadrature8<double>(lb, ub, f, min_points);
+        default: return num::quadrature9<double>(lb, ub, f, min_points);
+    }
+}
+
+}  // extern "C"
