@@ -1,0 +1,223 @@
+"""Benchmark: DCS evaluations per second (all 4 processes) on one MI355X.
+
+    python bench.py [--steps K] [--warmup W] [--n PAIRS] [--no-cpu]
+
+A step is one fused FP64 evaluation of bremsstrahlung, pair production,
+photonuclear and ionisation over N = 2^24 (K, q) pairs of standard rock (muon),
+inputs resident in HBM. `value` is pairs evaluated per second (each pair = 4
+DCS values). The same JSON line also reports the mixed-precision FP32 form,
+the batched 48-element persistent launch and (when trained) the surrogate,
+each with its max relative error against the FP64 GPU result over every pair,
+the roofline of the FP64 kernel against the FP64 vector peak, and the CPU
+baseline (the reference's OpenMP path on this host's cores).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+CPU_THREADS = 16  # the host CPU share of one GPU on the benchmark boxes
+os.environ.setdefault("OMP_NUM_THREADS", str(CPU_THREADS))
+
+FP64_VECTOR_PEAK_TFLOPS = 78.6
+METRIC = "DCS evals/sec (4 processes) FP64 vs FP32 vs surrogate at stated max rel err"
+
+
+def parse() -> argparse.Namespace:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=1 << 24)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-extras", action="store_true", help="only the FP64 headline")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="after the run, one plain device copy of BYTES (scales rocprof FETCH/WRITE_SIZE)")
+    return ap.parse_args()
+
+
+def timed_launches(fn, steps: int, warmup: int, stream):
+    """Per-launch durations from HIP events on the launch stream, plus the
+    wall time of the whole timed region (synchronised on both sides)."""
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    t0 = time.perf_counter()
+    for s, e in ev:
+        s.record(stream)
+        fn()
+        e.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    durs = [s.elapsed_time(e) for s, e in ev]
+    return wall, sum(durs) / len(durs)
+
+
+def max_rel_vs(got, ref) -> dict:
+    import torch
+    out = {}
+    for name, g, r in zip(("bremsstrahlung", "pair_production", "photonuclear", "ionisation"), got, ref):
+        g = g.double()
+        nz = r != 0
+        zero_mismatch = int(torch.count_nonzero((g != 0) != nz).item())
+        rel = torch.abs(g[nz] / r[nz] - 1)
+        out[name] = {"max_rel": float(rel.max().item()) if rel.numel() else 0.0, "zero_mismatch": zero_mismatch}
+    return out
+
+
+def read_traffic(kernel_prefix: str):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary, or None."""
+    path = os.path.join(ROOT, "profiles", "dcs_eval_f64_2e24_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    for line in open(path):
+        if ":" in line and line.split(":")[0].strip() in ("hbm_bytes_per_launch",):
+            vals["hbm"] = float(line.split(":")[1].split()[0])
+    return vals.get("hbm")
+
+
+def cpu_baseline(n_total: int) -> dict:
+    """The reference's OpenMP path (oracle/_ref, compiled from the reference
+    headers) or, where it is absent, the restated oracle, on this host."""
+    import numpy as np
+
+    import oracle
+    from noa_amd import workload
+    from noa_amd.physics import MUON_MASS, STANDARD_ROCK
+    ref = oracle.load_ref()
+    kind = "reference" if ref is not None else "port"
+    n = min(n_total, 1 << 22)
+    K, q = workload.pairs(n, seed=0)
+
+    def run(m):
+        if kind == "reference":
+            for p in range(4):
+                oracle.ref_map(p, K[:m], q[:m], STANDARD_ROCK, MUON_MASS, parallel=True)
+        else:
+            oracle.dcs_map_all(K[:m], q[:m], STANDARD_ROCK, MUON_MASS, threads=CPU_THREADS)
+
+    run(1 << 14)  # warm-up (thread team, page faults)
+    t = time.perf_counter()
+    run(n)
+    dt = time.perf_counter() - t
+    return {"value": n / dt, "unit": "evals/s", "cores": int(os.environ.get("OMP_NUM_THREADS", CPU_THREADS)),
+            "kind": kind,
+            "sample": f"first {n} pairs of the same seeded workload (2^{int(np.log2(n))}), all 4 processes, "
+                      f"one OpenMP map per process as dcs::pvmap does, {dt:.2f} s"}
+
+
+def main() -> None:
+    args = parse()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on one GPU only: DCS evaluation needs no exchange between GPUs, and this build "
+              "measures the single-GPU kernel (see DESIGN.md for how the path shards).", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from noa_amd import dcs, workload
+    from noa_amd.physics import MUON_MASS, STANDARD_ROCK
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    stream = torch.cuda.current_stream(dev)
+    n = args.n
+    K = torch.empty(n, dtype=torch.float64, device=dev)
+    q = torch.empty_like(K)
+    workload.fill_device(K, q, seed=0)
+    outs = tuple(torch.empty_like(K) for _ in range(4))
+    wall, kern_ms = timed_launches(lambda: dcs.eval_f64(K, q, STANDARD_ROCK, MUON_MASS, *outs),
+                                   args.steps, args.warmup, stream)
+    value = n * args.steps / wall
+
+    cm = json.load(open(os.path.join(ROOT, "tests", "golden", "costmodel.json")))
+    flops_pair = cm["fp64_flops_per_pair_total"]
+    achieved = flops_pair * n / (kern_ms * 1e-3) / 1e12
+    traffic = read_traffic("dcs_eval_kernel")
+    result = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded counter-based generator, on device)",
+        "config": {
+            "workload": "standard rock (Z=11, A=22 g/mol, I=0.1364e-6 GeV), muon mass 0.10565839 GeV, "
+                        f"N={n} (K,q) pairs, K log-uniform [1e-3,1e6] GeV, nu log-uniform [1e-6,1), q=nu*K, "
+                        "seed 0; all 4 processes fused in one launch",
+            "n_pairs": n,
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "valu",
+            "achieved": achieved,
+            "peak": FP64_VECTOR_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / FP64_VECTOR_PEAK_TFLOPS,
+            "traffic": traffic,
+            "kernel": "dcs_eval_kernel<F64,2048>",
+            "kernel_ms": kern_ms,
+            "algorithmic_flops_per_pair": flops_pair,
+        },
+    }
+
+    if not args.no_extras:
+        ref = outs
+        forms = {}
+        o32 = tuple(torch.empty(n, dtype=torch.float32, device=dev) for _ in range(4))
+        w32, k32 = timed_launches(lambda: dcs.eval_f32mixed(K, q, STANDARD_ROCK, MUON_MASS, *o32),
+                                  args.steps, args.warmup, stream)
+        forms["f32mixed"] = {"evals_per_s": n * args.steps / w32, "kernel_ms": k32,
+                             "speedup_vs_f64": wall / w32, "max_rel_err_vs_f64": max_rel_vs(o32, ref)}
+        del o32
+        try:
+            from noa_amd import surrogate
+            forms["surrogate"] = surrogate.bench_entry(K, q, ref, args.steps, args.warmup, stream, wall)
+        except (ImportError, FileNotFoundError) as e:
+            forms["surrogate"] = {"skipped": str(e)}
+        del K, q, outs, ref
+        torch.cuda.empty_cache()
+        from noa_amd.batch import BatchPlan
+        elements, offsets = workload.element_batch(48, seed=0)
+        nb = int(offsets[-1])
+        Kb = torch.empty(nb, dtype=torch.float64, device=dev)
+        qb = torch.empty_like(Kb)
+        workload.fill_device_batch(Kb, qb, offsets, seed=0)
+        for prec in ("f64", "f32mixed"):
+            plan = BatchPlan(elements, offsets, Kb, qb, MUON_MASS, precision=prec)
+            wb, kb = timed_launches(plan.launch, args.steps, args.warmup, stream)
+            forms[f"batched48_{prec}"] = {"evals_per_s": nb * args.steps / wb, "kernel_ms": kb,
+                                          "n_pairs": nb, **plan.info()}
+            plan.close()
+        result["forms"] = forms
+
+    if args.calibrate_copy:
+        src = torch.ones(args.calibrate_copy // 8, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        result["calibrate_copy_bytes"] = args.calibrate_copy
+
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline(n)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -46,6 +46,9 @@ __global__ __launch_bounds__(WG) void dcs_batch_kernel(const double* __restrict_
                                                        uint32_t* counter, Outs<typename P::out_t> o) {
     __shared__ TileLists<TILE> s;
     __shared__ uint32_t s_item;
+    __shared__ double fm_block[fm::TABLE_DOUBLES];
+    load_fm_tables(fm_block);  // published by the barrier below
+    const fm::Tab tab = fm::tab_from_block(fm_block);
     for (;;) {
         if (threadIdx.x == 0) s_item = atomicAdd(counter, 1u);
         __syncthreads();
@@ -58,7 +61,7 @@ __global__ __launch_bounds__(WG) void dcs_batch_kernel(const double* __restrict_
         // through the constant address space so its fields stay scalar loads
         using ConstPtr = const __attribute__((address_space(4))) DcsConsts*;
         const DcsConsts& c = *(const DcsConsts*)((ConstPtr)(consts + w.element));
-        run_tile<P, TILE>(w.begin, w.begin + w.len, K, q, c, o, s);  // ends with a barrier
+        run_tile<P, TILE>(w.begin, w.begin + w.len, K, q, c, o, s, tab);  // ends with a barrier
     }
 }
 

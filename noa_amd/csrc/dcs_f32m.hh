@@ -66,7 +66,7 @@ struct F32M {
     static constexpr bool has_redo = true;
 
     // Screening sum Z Phi_n + Phi_e of physics.hh:135-148 in FP64.
-    static __device__ float brems_screening_f64(double K, double q, const DcsConsts& c) {
+    static __device__ float brems_screening_f64(double K, double q, const DcsConsts& c, const fm::Tab& t) {
 #pragma clang fp contract(off)
         const double me = ELECTRON_MASS;
         const double sqrte = SQRTE_BREMS;
@@ -74,17 +74,17 @@ struct F32M {
         const double qe_max = E / (1. + c.b_half_m2 / (me * E));
         const double nu = q / E;
         const double delta = c.b_half_m2 / E * nu / (1. - nu);
-        double Phi_n = log(c.b_BZ_n * (c.mass + delta * c.b_Dn_sqrte_m2) / (c.b_D_n * (me + delta * sqrte * c.b_BZ_n)));
+        double Phi_n = fm::log(c.b_BZ_n * (c.mass + delta * c.b_Dn_sqrte_m2) / (c.b_D_n * (me + delta * sqrte * c.b_BZ_n)), t);
         if (Phi_n < 0.) Phi_n = 0.;
         double Phi_e = 0.;
         if (q < qe_max) {
-            Phi_e = log(c.b_BZe_mass / ((1. + delta * c.b_phie_factor) * (me + delta * sqrte * c.b_BZ_e)));
+            Phi_e = fm::log(c.b_BZe_mass / ((1. + delta * c.b_phie_factor) * (me + delta * sqrte * c.b_BZ_e)), t);
             if (Phi_e < 0.) Phi_e = 0.;
         }
         return float(c.Zd * Phi_n + Phi_e);
     }
 
-    static __device__ __forceinline__ float brems(double K, double q, const DcsConsts& c) {
+    static __device__ __forceinline__ float brems(double K, double q, const DcsConsts& c, const fm::Tab& t) {
         const float me = float(ELECTRON_MASS);
         const float sqrte = float(SQRTE_BREMS);
         const double E = K + c.mass;
@@ -104,12 +104,12 @@ struct F32M {
         }
         const float Zf = float(c.Zd);
         float S = Zf * Phi_n + Phi_e;
-        if (S < 0.05f * (Zf + 1.f)) S = brems_screening_f64(K, q, c);
+        if (S < 0.05f * (Zf + 1.f)) S = brems_screening_f64(K, q, c, t);
         const float F = 4.f / 3.f * fdiv(omnu, nu) + nu;
         return float(c.f_brems_c) * S * F;
     }
 
-    static __device__ __forceinline__ float ion(double K, double q, const DcsConsts& c) {
+    static __device__ __forceinline__ float ion(double K, double q, const DcsConsts& c, const fm::Tab& t) {
         const double me = ELECTRON_MASS;
         const double P2 = K * (K + c.i_2m);
         const double E = K + c.mass;
@@ -139,7 +139,7 @@ struct F32M {
     // edges, e.g. q -> 4 m_e) exceeds PAIR_REDO_TOL of the integral.
     static constexpr float PAIR_REDO_TOL = 2e-6f;
 
-    static __device__ float pair(double K, double q, const DcsConsts& c, bool& redo) {
+    static __device__ float pair(double K, double q, const DcsConsts& c, bool& redo, const fm::Tab& t) {
         // FP64 set-up: the integration bound exactly as dcs.hh:169-176
         double tmin_d;
         {
@@ -148,7 +148,7 @@ struct F32M {
             const double x0 = 4. * ELECTRON_MASS / q;
             const double x1 = 6. / (gamma * (gamma - q / c.mass));
             const double argmin = (x0 + 2. * (1. - x0) * x1) / (1. + (1. - x1) * sqrt(1. - x0));
-            tmin_d = log(argmin);
+            tmin_d = fm::log(argmin, t);
         }
         const double E = K + c.mass;
         const float Ef = float(E);
@@ -230,7 +230,7 @@ struct F32M {
         return F64::photo_active(K, q, c);
     }
 
-    static __device__ float photo(double K, double q, const DcsConsts& c, bool& redo) {
+    static __device__ float photo(double K, double q, const DcsConsts& c, bool& redo, const fm::Tab& t) {
         redo = false;
         const double M = PHOTO_M;
         const double mpi = PHOTO_MPI;
@@ -241,8 +241,8 @@ struct F32M {
             const double y = q / E;
             const double Q2min = c.ph_m2 * y * y / (1. - y);
             Q2max_d = 2.0 * M * (q - mpi) - mpi * mpi;
-            lQ2max_d = log(Q2max_d);
-            dpQ2_d = lQ2max_d - log(Q2min);
+            lQ2max_d = fm::log(Q2max_d, t);
+            dpQ2_d = lQ2max_d - fm::log(Q2min, t);
         }
         const float Ef = float(E);
         const float qf = float(q);
@@ -271,8 +271,8 @@ struct F32M {
             const float x = Q2 * inv2Mq;
             const float omx = D * inv2Mq;
             // ALLM97 F2 (dcs.hh:261-307)
-            const float t = log1pf(fdiv(log1pf(fdiv(Q2, float(ALLM_Q02))), L0));
-            const float lnt = __logf(t);
+            const float tl = log1pf(fdiv(log1pf(fdiv(Q2, float(ALLM_Q02))), L0));
+            const float lnt = __logf(tl);
             const float cP = 0.28067f + (0.28067f - 0.22291f) * (fdiv(1.f, 1.f + __expf(2.1979f * lnt)) - 1.f);
             const float aP = -0.0808f + (-0.0808f + 0.44812f) * (fdiv(1.f, 1.f + __expf(1.1709f * lnt)) - 1.f);
             const float bP = 0.36292f + 1.8917f * __expf(1.8439f * lnt);
@@ -290,7 +290,7 @@ struct F32M {
             // decided with the FP64 Bjorken x of this node.
             bool below_hi = x < 0.04f;
             if (fabsf(x - 0.04f) < 1e-7f)
-                below_hi = 0.5 * exp(lQ2max_d - 0.5 * dpQ2_d * (1.0 - GL9_X[j])) / (M * q) < 0.04;
+                below_hi = 0.5 * fm::exp(lQ2max_d - 0.5 * dpQ2_d * (1.0 - GL9_X[j]), t) / (M * q) < 0.04;
             float a = 1.f;
             if (x < 0.0014f)
                 a = float(c.ph_a_low);

@@ -2,7 +2,8 @@
 //
 // Every expression keeps the reference's evaluation order and FMA
 // contraction is off in every function body, so the only difference from the
-// reference's CPU path is device libm (ocml log/exp/log10) against glibc.
+// reference's CPU path is the device log/exp (the table-driven ones of
+// fastmath.hh, ~0.5 ulp; ocml for log10 and sqrt) against glibc.
 // Launch invariants come precomputed in DcsConsts.
 //   bremsstrahlung   src/noa/pms/physics.hh:114-153
 //   pair production  src/noa/pms/dcs.hh:144-258 (8-node rule numerics.hh:110-129)
@@ -13,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include "dcs_consts.hh"
+#include "fastmath.hh"
 
 // FMA contraction is switched off inside each function body (block scope),
 // so including this header does not change the rest of a translation unit.
@@ -38,7 +40,7 @@ struct F64 {
     static constexpr bool has_redo = false;
 
     // physics.hh:114-153
-    static __device__ __forceinline__ double brems(double K, double q, const DcsConsts& c) {
+    static __device__ __forceinline__ double brems(double K, double q, const DcsConsts& c, const fm::Tab& t) {
 #pragma clang fp contract(off)
         const double me = ELECTRON_MASS;
         const double sqrte = SQRTE_BREMS;
@@ -47,12 +49,12 @@ struct F64 {
         const double qe_max = E / (1. + c.b_half_m2 / (me * E));
         const double nu = q / E;
         const double delta = delta_factor * nu / (1. - nu);
-        double Phi_n = log(c.b_BZ_n * (c.mass + delta * c.b_Dn_sqrte_m2) /
-                           (c.b_D_n * (me + delta * sqrte * c.b_BZ_n)));
+        double Phi_n = fm::log(c.b_BZ_n * (c.mass + delta * c.b_Dn_sqrte_m2) /
+                           (c.b_D_n * (me + delta * sqrte * c.b_BZ_n)), t);
         if (Phi_n < 0.) Phi_n = 0.;
         double Phi_e = 0.;
         if (q < qe_max) {
-            Phi_e = log(c.b_BZe_mass / ((1. + delta * c.b_phie_factor) * (me + delta * sqrte * c.b_BZ_e)));
+            Phi_e = fm::log(c.b_BZe_mass / ((1. + delta * c.b_phie_factor) * (me + delta * sqrte * c.b_BZ_e)), t);
             if (Phi_e < 0.) Phi_e = 0.;
         }
         const double dcs = c.b_dcs_factor * (c.Zd * Phi_n + Phi_e) * (4. / 3. * (1. / nu - 1.) + nu);
@@ -60,7 +62,7 @@ struct F64 {
     }
 
     // dcs.hh:408-443
-    static __device__ __forceinline__ double ion(double K, double q, const DcsConsts& c) {
+    static __device__ __forceinline__ double ion(double K, double q, const DcsConsts& c, const fm::Tab& t) {
 #pragma clang fp contract(off)
         const double me = ELECTRON_MASS;
         const double P2 = K * (K + c.i_2m);
@@ -74,8 +76,8 @@ struct F64 {
         const double cs = 1.535336E-05 * E * c.Zd / c.A * (a0 + 1. / q * (a1 + a2 / q));
         double Delta = 0.;
         if (K >= c.i_Kthr) {
-            const double L1 = log(1. + 2. * q / me);
-            Delta = 1.16141E-03 * L1 * (log(4. * E * (E - q) / c.i_m2) - L1);
+            const double L1 = fm::log(1. + 2. * q / me, t);
+            Delta = 1.16141E-03 * L1 * (fm::log(4. * E * (E - q) / c.i_m2, t) - L1);
         }
         return cs * (1. + Delta);
     }
@@ -93,7 +95,7 @@ struct F64 {
     }
 
     // dcs.hh:144-258 for a pair that passed pair_active.
-    static __device__ double pair(double K, double q, const DcsConsts& c) {
+    static __device__ double pair(double K, double q, const DcsConsts& c, const fm::Tab& t) {
 #pragma clang fp contract(off)
         const double me = ELECTRON_MASS;
         const double nu = q / (K + c.mass);
@@ -103,13 +105,13 @@ struct F64 {
         const double x0 = 4. * me / q;
         const double x1 = 6. / (gamma * (gamma - q / c.mass));
         const double argmin = (x0 + 2. * (1. - x0) * x1) / (1. + (1. - x1) * sqrt(1. - x0));
-        const double tmin = log(argmin);
+        const double tmin = fm::log(argmin, t);
 
         double I = 0.;
 #pragma unroll 1
         for (int j = 0; j < 8; j++) {
-            const double t = GL8_X[j];
-            const double eps = exp(t * tmin);
+            const double tj = GL8_X[j];
+            const double eps = fm::exp(tj * tmin, t);
             const double rho = 1. - eps;
             const double rho2 = rho * rho;
             const double rho21 = eps * (2. - eps);
@@ -119,25 +121,25 @@ struct F64 {
             if (xi >= 1E+03)
                 Be = 0.5 * xi_i * ((3. - rho2) + 2. * beta * (1. + rho2));
             else
-                Be = ((2. + rho2) * (1. + beta) + xi * (3. + rho2)) * log(1. + xi_i) +
+                Be = ((2. + rho2) * (1. + beta) + xi * (3. + rho2)) * fm::log(1. + xi_i, t) +
                      (rho21 - beta) / (1. + xi) - 3. - rho2;
             const double Ye = (5. - rho2 + 4. * beta * (1. + rho2)) /
-                              (2. * (1. + 3. * beta) * log(3. + xi_i) - rho2 - 2. * beta * (2. - rho2));
+                              (2. * (1. + 3. * beta) * fm::log(3. + xi_i, t) - rho2 - 2. * beta * (2. - rho2));
             const double xe = (1. + xi) * (1. + Ye);
             const double cLi = c.p_cL / rho21;
-            const double Le = log(c.p_AZ13 * sqrt(xe) * q / (q + cLi * xe)) - 0.5 * log(1. + c.p_cLe * xe);
+            const double Le = fm::log(c.p_AZ13 * sqrt(xe) * q / (q + cLi * xe), t) - 0.5 * fm::log(1. + c.p_cLe * xe, t);
             double Phi_e = Be * Le;
             if (Phi_e < 0.) Phi_e = 0.;
             double Bmu;
             if (xi <= 1E-03)
                 Bmu = 0.5 * xi * (5. - rho2 + beta * (3. + rho2));
             else
-                Bmu = ((1. + rho2) * (1. + 1.5 * beta) - xi_i * (1. + 2. * beta) * rho21) * log(1. + xi) +
+                Bmu = ((1. + rho2) * (1. + 1.5 * beta) - xi_i * (1. + 2. * beta) * rho21) * fm::log(1. + xi, t) +
                       xi * (rho21 - beta) / (1. + xi) + (1. + 2. * beta) * rho21;
             const double Ymu = (4. + rho2 + 3. * beta * (1. + rho2)) /
-                               ((1. + rho2) * (1.5 + 2. * beta) * log(3. + xi) + 1. - 1.5 * rho2);
+                               ((1. + rho2) * (1.5 + 2. * beta) * fm::log(3. + xi, t) + 1. - 1.5 * rho2);
             const double xmu = (1. + xi) * (1. + Ymu);
-            const double Lmu = log(c.p_rAZ13 * q / (c.p_15Z13 * (q + cLi * xmu)));
+            const double Lmu = fm::log(c.p_rAZ13 * q / (c.p_15Z13 * (q + cLi * xmu)), t);
             double Phi_mu = Bmu * Lmu;
             if (Phi_mu < 0.) Phi_mu = 0.;
             const double f = -(Phi_e + Phi_mu / c.p_r2) * (1. - rho) * tmin;
@@ -148,11 +150,11 @@ struct F64 {
         if (gamma <= 35.) {
             zeta = 0.;
         } else {
-            zeta = 0.073 * log(gamma / (1. + c.p_g1 * gamma * c.p_Z13 * c.p_Z13)) - 0.26;
+            zeta = 0.073 * fm::log(gamma / (1. + c.p_g1 * gamma * c.p_Z13 * c.p_Z13), t) - 0.26;
             if (zeta <= 0.)
                 zeta = 0.;
             else
-                zeta /= 0.058 * log(gamma / (1. + c.p_g2 * gamma * c.p_Z13)) - 0.14;
+                zeta /= 0.058 * fm::log(gamma / (1. + c.p_g2 * gamma * c.p_Z13), t) - 0.14;
         }
         const double E = K + c.mass;
         const double dcs = c.p_dcs_c * (c.Zd + zeta) * (E - q) * I / (q * E);
@@ -172,7 +174,7 @@ struct F64 {
     }
 
     // dcs.hh:261-405 for a pair that passed photo_active.
-    static __device__ double photo(double K, double q, const DcsConsts& c) {
+    static __device__ double photo(double K, double q, const DcsConsts& c, const fm::Tab& t) {
 #pragma clang fp contract(off)
         const double M = PHOTO_M;
         const double mpi = PHOTO_MPI;
@@ -180,8 +182,8 @@ struct F64 {
         const double y = q / E;
         const double Q2min = c.ph_m2 * y * y / (1. - y);
         const double Q2max = 2.0 * M * (q - mpi) - mpi * mpi;
-        const double pQ2min = log(Q2min);
-        const double pQ2max = log(Q2max);
+        const double pQ2min = fm::log(Q2min, t);
+        const double pQ2max = fm::log(Q2max, t);
         const double dpQ2 = pQ2max - pQ2min;
         const double pQ2c = 0.5 * (pQ2max + pQ2min);
         const double hd = 0.5 * dpQ2;
@@ -191,38 +193,38 @@ struct F64 {
         double ds = 0.;
 #pragma unroll 1
         for (int j = 0; j < 9; j++) {
-            const double Q2 = exp(pQ2c + hd * GL9_X[j]);
+            const double Q2 = fm::exp(pQ2c + hd * GL9_X[j], t);
             // dcs_photonuclear_d2 (dcs.hh:335-355)
             const double x = 0.5 * Q2 / Mq;
             // dcs_photonuclear_f2_allm (dcs.hh:261-307)
             const double M2 = 0.8803505929;
             const double W2 = M2 + Q2 * (1.0 / x - 1.0);
-            const double t = log(log((Q2 + ALLM_Q02) / ALLM_LAMBDA2) / c.ph_L0);
+            const double tl = fm::log(fm::log((Q2 + ALLM_Q02) / ALLM_LAMBDA2, t) / c.ph_L0, t);
             const double xP = (Q2 + 49.457) / (Q2 + 49.457 + W2 - M2);
             const double xR = (Q2 + 0.15052) / (Q2 + 0.15052 + W2 - M2);
-            const double lnt = log(t);
-            const double cP = 0.28067 + (0.28067 - 0.22291) * (1.0 / (1.0 + exp(2.1979 * lnt)) - 1.0);
-            const double aP = -0.0808 + (-0.0808 - -0.44812) * (1.0 / (1.0 + exp(1.1709 * lnt)) - 1.0);
-            const double bP = 0.36292 + 1.8917 * exp(1.8439 * lnt);
-            const double cR = 0.80107 + 0.97307 * exp(3.4942 * lnt);
-            const double aR = 0.58400 + 0.37888 * exp(2.6063 * lnt);
-            const double bR = 0.01147 + 3.7582 * exp(0.49338 * lnt);
-            const double l1x = log(1. - x);
-            const double F2P = cP * exp(aP * log(xP) + bP * l1x);
-            const double F2R = cR * exp(aR * log(xR) + bR * l1x);
+            const double lnt = fm::log(tl, t);
+            const double cP = 0.28067 + (0.28067 - 0.22291) * (1.0 / (1.0 + fm::exp(2.1979 * lnt, t)) - 1.0);
+            const double aP = -0.0808 + (-0.0808 - -0.44812) * (1.0 / (1.0 + fm::exp(1.1709 * lnt, t)) - 1.0);
+            const double bP = 0.36292 + 1.8917 * fm::exp(1.8439 * lnt, t);
+            const double cR = 0.80107 + 0.97307 * fm::exp(3.4942 * lnt, t);
+            const double aR = 0.58400 + 0.37888 * fm::exp(2.6063 * lnt, t);
+            const double bR = 0.01147 + 3.7582 * fm::exp(0.49338 * lnt, t);
+            const double l1x = fm::log(1. - x, t);
+            const double F2P = cP * fm::exp(aP * fm::log(xP, t) + bP * l1x, t);
+            const double F2R = cR * fm::exp(aR * fm::log(xR, t) + bR * l1x, t);
             const double F2p = Q2 / (Q2 + 0.31985) * (F2P + F2R);
             // dcs_photonuclear_f2a_drss (dcs.hh:310-319)
             double a = 1.0;
             if (x < 0.0014)
                 a = c.ph_a_low;
             else if (x < 0.04)
-                a = exp((0.069 * log10(x) + 0.097) * c.ph_logA);
+                a = fm::exp((0.069 * log10(x) + 0.097) * c.ph_logA, t);
             const double F2A = c.ph_halfA * a * (2.0 + x * (-1.85 + x * (2.45 + x * (-2.35 + x)))) * F2p;
             // dcs_photonuclear_r_whitlow (dcs.hh:322-332)
             double q2 = Q2;
             if (Q2 < 0.3) q2 = 0.3;
             const double theta = 1. + 12.0 * q2 / (1.0 + q2) * 0.015625 / (0.015625 + x * x);
-            const double R = 0.635 / log(q2 / 0.04) * theta + 0.5747 / q2 - 0.3534 / (0.09 + q2 * q2);
+            const double R = 0.635 / fm::log(q2 / 0.04, t) * theta + 0.5747 / q2 - 0.3534 / (0.09 + q2 * q2);
             const double dds =
                 (1. - y + 0.5 * (1. - c.ph_2m2 / Q2) * (y * y + Q2 / EE) / (1. + R)) / (Q2 * Q2) -
                 0.25 / (EE * Q2);
@@ -233,11 +235,11 @@ struct F64 {
     }
 
     // The tile runner's interface; FP64 results are final (no recomputation).
-    static __device__ __forceinline__ double pair(double K, double q, const DcsConsts& c, bool&) {
-        return pair(K, q, c);
+    static __device__ __forceinline__ double pair(double K, double q, const DcsConsts& c, bool&, const fm::Tab& t) {
+        return pair(K, q, c, t);
     }
-    static __device__ __forceinline__ double photo(double K, double q, const DcsConsts& c, bool&) {
-        return photo(K, q, c);
+    static __device__ __forceinline__ double photo(double K, double q, const DcsConsts& c, bool&, const fm::Tab& t) {
+        return photo(K, q, c, t);
     }
 };
 

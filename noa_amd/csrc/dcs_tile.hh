@@ -22,6 +22,7 @@
 
 #include "dcs_consts.hh"
 #include "dcs_f64.hh"
+#include "fastmath.hh"
 
 namespace noa_hip {
 
@@ -62,12 +63,19 @@ __device__ __forceinline__ void wave_append(bool pred, uint32_t* counter, uint16
     if (pred) list[base + rank] = uint16_t(value);
 }
 
-// Evaluates pairs [base, end) (end - base <= TILE) of one element. Must be
-// called by all WG threads of the workgroup.
+// Copies the log/exp tables into an LDS block (visible after the next barrier).
+__device__ __forceinline__ void load_fm_tables(double* block) {
+    for (int k = threadIdx.x; k < fm::TABLE_DOUBLES; k += WG) block[k] = fm::table_entry(k);
+}
+
+// Evaluates pairs [base, end) (end - base <= TILE) of one element with the
+// log/exp tables `t` in LDS. Must be called by all WG threads of the
+// workgroup.
 template <class P, int TILE>
 __device__ __forceinline__ void run_tile(int64_t base, int64_t end, const double* __restrict__ K,
                                          const double* __restrict__ q, const DcsConsts& c,
-                                         const Outs<typename P::out_t>& o, TileLists<TILE>& s) {
+                                         const Outs<typename P::out_t>& o, TileLists<TILE>& s,
+                                         const fm::Tab& t) {
     using out_t = typename P::out_t;
     constexpr int ITEMS = TILE / WG;
     const int tid = threadIdx.x;
@@ -99,8 +107,8 @@ __device__ __forceinline__ void run_tile(int64_t base, int64_t end, const double
         }
         bool pa = false, pha = false;
         if (valid) {
-            if (o.b) o.b[gi] = P::brems(k, qq, c);
-            if (o.i) o.i[gi] = P::ion(k, qq, c);
+            if (o.b) o.b[gi] = P::brems(k, qq, c, t);
+            if (o.i) o.i[gi] = P::ion(k, qq, c, t);
             if (want_pair) {
                 pa = P::pair_active(k, qq, c);
                 if (!pa) o.p[gi] = out_t(0);
@@ -125,12 +133,12 @@ __device__ __forceinline__ void run_tile(int64_t base, int64_t end, const double
         if (v < nph) {
             tag = s.photo[v];
             const int64_t g = base + tag;
-            o.ph[g] = P::photo(K[g], q[g], c, redo);
+            o.ph[g] = P::photo(K[g], q[g], c, redo, t);
             tag |= REDO_PHOTO;
         } else {
             tag = s.pair[v - nph];
             const int64_t g = base + tag;
-            o.p[g] = P::pair(K[g], q[g], c, redo);
+            o.p[g] = P::pair(K[g], q[g], c, redo, t);
         }
         if (P::has_redo) wave_append(redo, &s.n_redo, s.redo, tag);
     }
@@ -144,9 +152,9 @@ __device__ __forceinline__ void run_tile(int64_t base, int64_t end, const double
             const uint32_t tag = s.redo[v];
             const int64_t g = base + (tag & (REDO_PHOTO - 1));
             if (tag & REDO_PHOTO)
-                o.ph[g] = out_t(F64::photo(K[g], q[g], c));
+                o.ph[g] = out_t(F64::photo(K[g], q[g], c, t));
             else
-                o.p[g] = out_t(F64::pair(K[g], q[g], c));
+                o.p[g] = out_t(F64::pair(K[g], q[g], c, t));
         }
     }
     __syncthreads();  // the lists are reused by the next tile of a persistent workgroup

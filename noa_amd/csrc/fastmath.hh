@@ -1,0 +1,125 @@
+// Table-driven FP64 log and exp (Tang's method) for the DCS kernels.
+//
+// ocml's FP64 log is ~98 VALU instructions and exp ~42 on gfx950, and the DCS
+// path makes ~52 log and ~24 exp calls per (K, q) pair, so libm dominates the
+// VALU issue budget. These versions handle positive normal finite arguments
+// (log) and |x| < 708 (exp) with ~30 and ~22 instructions plus LDS table
+// reads, at about 0.5 ulp; anything else falls through to libm.
+//
+// log x = k ln2 + log(1/invc_i) + log1p(r),  r = z invc_i - 1, |r| < 2^-7,
+//   with x = 2^k z, z in [0.6875, 1.375) and i from the top 7 mantissa bits
+//   (invc = 1 on the two subintervals that touch 1, so r = z - 1 exactly and
+//   log x keeps full relative accuracy as x -> 1); k LN2_HI + logc_hi is
+//   exact (both multiples of 2^-43), the sum with r is a TwoSum, and
+//   log1p(r) - r is a degree-9 Taylor polynomial.
+// exp x = 2^m 2^(j/128) exp(r), k = rint(128 x / ln2) = 128 m + j,
+//   r = x - k ln2/128 (two FMAs), exp(r) - 1 a degree-5 Taylor polynomial.
+//
+// The header is compiled by hipcc for the kernels and by a host compiler for
+// the CPU accuracy tests; NOA_HD carries the qualifiers.
+#pragma once
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define NOA_HD __host__ __device__
+#define NOA_FM_TABLE static __constant__ const
+#else
+#define NOA_HD
+#define NOA_FM_TABLE static const
+#endif
+
+#include "fastmath_tables.hh"
+
+namespace noa_hip {
+namespace fm {
+
+constexpr uint64_t LOG_OFF = 0x3fe6000000000000ull;  // bits of 0.6875
+constexpr int TABLE_DOUBLES = 3 * LOG_N + 2 * EXP_N;
+
+// Pointers to one copy of the tables (LDS in the kernels).
+struct Tab {
+    const double* invc;
+    const double* lhi;
+    const double* llo;
+    const double* ehi;
+    const double* elo;
+};
+
+// Layout of the flat table block: invc | logc_hi | logc_lo | exp_hi | exp_lo.
+NOA_HD inline Tab tab_from_block(const double* b) {
+    return Tab{b, b + LOG_N, b + 2 * LOG_N, b + 3 * LOG_N, b + 3 * LOG_N + EXP_N};
+}
+
+// Source of the flat block, element k.
+NOA_HD inline double table_entry(int k) {
+    if (k < LOG_N) return LOG_INVC[k];
+    if (k < 2 * LOG_N) return LOG_LOGC_HI[k - LOG_N];
+    if (k < 3 * LOG_N) return LOG_LOGC_LO[k - 2 * LOG_N];
+    if (k < 3 * LOG_N + EXP_N) return EXP_T_HI[k - 3 * LOG_N];
+    return EXP_T_LO[k - 3 * LOG_N - EXP_N];
+}
+
+NOA_HD inline uint64_t as_bits(double x) {
+    uint64_t u;
+    std::memcpy(&u, &x, sizeof(u));
+    return u;
+}
+
+NOA_HD inline double from_bits(uint64_t u) {
+    double x;
+    std::memcpy(&x, &u, sizeof(x));
+    return x;
+}
+
+NOA_HD inline double log(double x, const Tab& t) {
+#pragma clang fp contract(off)
+    const uint64_t ix = as_bits(x);
+    // positive normal finite: 0x0010... <= ix < 0x7ff0...
+    if (ix - 0x0010000000000000ull >= 0x7fe0000000000000ull) return std::log(x);
+    const uint64_t tmp = ix - LOG_OFF;
+    const int i = int((tmp >> 45) & uint64_t(LOG_N - 1));
+    const int k = int(int64_t(tmp) >> 52);
+    const double z = from_bits(ix - (tmp & 0xfff0000000000000ull));
+    const double kd = double(k);
+    const double r = std::fma(z, t.invc[i], -1.0);
+    const double w = std::fma(kd, LN2_HI, t.lhi[i]);  // exact
+    const double s = w + r;                           // TwoSum(w, r)
+    const double bb = s - w;
+    const double err = (w - (s - bb)) + (r - bb);
+    double q = std::fma(r, 1.0 / 9.0, -1.0 / 8.0);
+    q = std::fma(q, r, 1.0 / 7.0);
+    q = std::fma(q, r, -1.0 / 6.0);
+    q = std::fma(q, r, 1.0 / 5.0);
+    q = std::fma(q, r, -1.0 / 4.0);
+    q = std::fma(q, r, 1.0 / 3.0);
+    q = std::fma(q, r, -0.5);
+    double lo = std::fma(kd, LN2_LO, t.llo[i]) + err;
+    lo = std::fma(r * r, q, lo);
+    return s + lo;
+}
+
+NOA_HD inline double exp(double x, const Tab& t) {
+#pragma clang fp contract(off)
+    if (!(std::fabs(x) < 708.0)) return std::exp(x);  // overflow, underflow, NaN
+    const double kd = std::rint(x * INV_LN2_N);
+    const int ki = int(kd);
+    double r = std::fma(-kd, LN2_N_HI, x);
+    r = std::fma(-kd, LN2_N_LO, r);
+    const int j = ki & (EXP_N - 1);
+    const int m = ki >> 7;  // floor(ki / 128)
+    double p = std::fma(r, 1.0 / 120.0, 1.0 / 24.0);
+    p = std::fma(p, r, 1.0 / 6.0);
+    p = std::fma(p, r, 0.5);
+    p = std::fma(p, r, 1.0);
+    p = p * r;
+    const double th = t.ehi[j];
+    const double y = th + std::fma(th, p, t.elo[j]);
+    return std::ldexp(y, m);
+}
+
+}  // namespace fm
+}  // namespace noa_hip

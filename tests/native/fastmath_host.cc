@@ -1,0 +1,28 @@
+// Host build of noa_amd/csrc/fastmath.hh for the CPU accuracy tests: the same
+// table-driven log/exp the kernels run, evaluated over arrays.
+#include "../../noa_amd/csrc/fastmath.hh"
+
+using noa_hip::fm::TABLE_DOUBLES;
+
+namespace {
+struct Block {
+    double v[TABLE_DOUBLES];
+    Block() {
+        for (int k = 0; k < TABLE_DOUBLES; k++) v[k] = noa_hip::fm::table_entry(k);
+    }
+};
+const Block& block() {
+    static Block b;
+    return b;
+}
+}  // namespace
+
+extern "C" void fm_log_batch(const double* x, double* y, long n) {
+    const auto t = noa_hip::fm::tab_from_block(block().v);
+    for (long i = 0; i < n; i++) y[i] = noa_hip::fm::log(x[i], t);
+}
+
+extern "C" void fm_exp_batch(const double* x, double* y, long n) {
+    const auto t = noa_hip::fm::tab_from_block(block().v);
+    for (long i = 0; i < n; i++) y[i] = noa_hip::fm::exp(x[i], t);
+}

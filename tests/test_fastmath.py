@@ -1,0 +1,78 @@
+"""CPU accuracy of the table-driven FP64 log/exp used by the kernels
+(noa_amd/csrc/fastmath.hh, compiled here for the host): error in ulps against
+60-digit decimal values, and agreement with the host libm."""
+import ctypes
+import os
+import subprocess
+from decimal import Decimal, getcontext
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+@pytest.fixture(scope="module")
+def fm(tmp_path_factory):
+    so = str(tmp_path_factory.mktemp("fm") / "libfm.so")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-Wno-unknown-pragmas", "-shared", "-fPIC",
+                    os.path.join(HERE, "native", "fastmath_host.cc"), "-o", so], check=True)
+    lib = ctypes.CDLL(so)
+    D = ctypes.POINTER(ctypes.c_double)
+    for f in (lib.fm_log_batch, lib.fm_exp_batch):
+        f.argtypes = [D, D, ctypes.c_long]
+
+    def call(f, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty_like(x)
+        f(x.ctypes.data_as(D), y.ctypes.data_as(D), x.size)
+        return y
+    return {"log": lambda x: call(lib.fm_log_batch, x), "exp": lambda x: call(lib.fm_exp_batch, x)}
+
+
+def ulp_err(got: np.ndarray, exact: list) -> np.ndarray:
+    out = []
+    for g, e in zip(got, exact):
+        ulp = Decimal(float(np.spacing(abs(float(e)))))
+        out.append(float(abs(Decimal(float(g)) - e) / ulp))
+    return np.array(out)
+
+
+def test_log_ulp_vs_exact(fm):
+    getcontext().prec = 60
+    rng = np.random.default_rng(7)
+    x = np.concatenate([10 ** rng.uniform(-300, 300, 1500), rng.uniform(0.5, 2.0, 1500),
+                        1 + rng.uniform(-1e-3, 1e-3, 500)])
+    got = fm["log"](x)
+    exact = [Decimal(float(v)).ln() for v in x]
+    err = ulp_err(got, exact)
+    assert err.max() < 0.6, err.max()
+
+
+def test_exp_ulp_vs_exact(fm):
+    getcontext().prec = 60
+    rng = np.random.default_rng(8)
+    x = np.concatenate([rng.uniform(-700, 700, 1500), rng.uniform(-1, 1, 1500), rng.uniform(-1e-6, 1e-6, 200)])
+    got = fm["exp"](x)
+    exact = [Decimal(float(v)).exp() for v in x]
+    err = ulp_err(got, exact)
+    assert err.max() < 0.6, err.max()
+
+
+def test_matches_libm_and_specials(fm):
+    rng = np.random.default_rng(9)
+    x = 10 ** rng.uniform(-30, 30, 200000)
+    d = np.abs(fm["log"](x) - np.log(x)) / np.spacing(np.abs(np.log(x)))
+    assert d.max() <= 1.0 and (d == 0).mean() > 0.999
+    y = rng.uniform(-700, 700, 200000)
+    d = np.abs(fm["exp"](y) - np.exp(y)) / np.spacing(np.exp(y))
+    # glibc's exp is itself a ~0.51 ulp routine (not correctly rounded)
+    assert d.max() <= 1.0 and (d == 0).mean() > 0.9
+    # outside the fast ranges the functions defer to libm
+    sp = np.array([0.0, -1.0, np.inf, np.nan, 1e-310, 5e-324])
+    with np.errstate(all="ignore"):
+        np.testing.assert_array_equal(fm["log"](sp), np.log(sp))
+    se = np.array([-745.0, -709.0, 709.5, 710.0, np.inf, -np.inf, np.nan])
+    with np.errstate(all="ignore"):
+        np.testing.assert_array_equal(fm["exp"](se), np.exp(se))
+    assert fm["log"](np.array([1.0]))[0] == 0.0 and fm["exp"](np.array([0.0]))[0] == 1.0
