@@ -1,0 +1,200 @@
+"""The CPU oracle is pinned before it is trusted (CPU only).
+
+1. bit-for-bit against golden vectors written by the reference itself
+   (tests/golden/dcs_golden.npz from oracle/_ref, the reference headers
+   compiled in place), including every threshold approached from both sides;
+2. bit-for-bit against oracle/_ref on fresh random inputs, serial and OpenMP
+   (where the reference sources were present to build it);
+3. the notebook's printed known answers (docs/pms/muon_dcs_calc.ipynb, 5 s.f.);
+4. what the mathematics fixes: Gauss-Legendre exactness on monomials, the
+   truncated literals of the 8- and 6-node rules, the small-nu limits of the
+   closed forms, non-negativity and the closed-form zero masks.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from noa_amd import workload
+from noa_amd.physics import AVOGADRO_NUMBER, ELECTRON_MASS, MUON_MASS, STANDARD_ROCK, AtomicElement
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "dcs_golden.npz")
+BLOCKS = ("rock_muon", "hydrogen_muon", "iron_muon", "uranium_muon", "rock_tau")
+MASS = {"muon": MUON_MASS, "tau": 1.77682}
+
+
+@pytest.fixture(scope="module")
+def golden():
+    return np.load(GOLDEN)
+
+
+def _element(g, name):
+    A, I, Z = g[f"{name}_element"]
+    return AtomicElement(A=float(A), I=float(I), Z=int(Z))
+
+
+@pytest.mark.parametrize("block", BLOCKS)
+def test_oracle_bitwise_equals_reference_golden(golden, block):
+    el_name, particle = block.split("_")
+    el = _element(golden, el_name)
+    K, q, exp = golden[f"{block}_K"], golden[f"{block}_q"], golden[f"{block}_dcs"]
+    got = oracle.dcs_map_all(K, q, el, MASS[particle])
+    for p in range(4):
+        assert np.array_equal(got[p].view(np.uint64), exp[p].view(np.uint64)), (block, p)
+
+
+def test_golden_covers_both_sides_of_every_exit(golden):
+    exp = golden["rock_muon_dcs"]
+    for p in (1, 2, 3):  # the three processes with kinematic exits
+        assert (exp[p] == 0).sum() > 100 and (exp[p] > 0).sum() > 100
+
+
+@pytest.mark.skipif(oracle.load_ref() is None, reason="reference sources not present to build oracle/_ref")
+@pytest.mark.parametrize("parallel", [False, True])
+def test_oracle_bitwise_equals_reference_build(parallel):
+    K, q = workload.pairs(1 << 15, seed=5)
+    for el in (STANDARD_ROCK, AtomicElement(A=1.00794, I=19.2e-9, Z=1), AtomicElement(A=207.2, I=823e-9, Z=82)):
+        for p in range(4):
+            a = oracle.dcs_map(p, K, q, el, MUON_MASS, threads=0 if parallel else 1)
+            b = oracle.ref_map(p, K, q, el, MUON_MASS, parallel=parallel)
+            assert np.array_equal(a.view(np.uint64), b.view(np.uint64)), (el.Z, p)
+
+
+def test_notebook_known_answers():
+    torch = pytest.importorskip("torch")
+    # docs/pms/muon_dcs_calc.ipynb cells 12, 18, 25, 28, 31 (JSON lines 174-175, 325, 535, 670, 741)
+    K = torch.linspace(1e-3, 1e6, 10000).double().numpy()
+    q = 0.0505 * K
+    expected = {
+        0: [3.5293e-04, 3.9395e-06, 4.0777e-06, 4.1341e-06, 4.1650e-06],
+        1: [0.0, 6.5366e-06, 7.3699e-06, 7.7919e-06, 8.0572e-06],
+        2: [0.0, 2.2912e-06, 2.1304e-06, 2.0719e-06, 2.0427e-06],
+        3: [0.0, 3.0168e-05, 1.5300e-05, 1.0284e-05, 7.7575e-06],
+    }
+    got = oracle.dcs_map_all(K[:5], q[:5], STANDARD_ROCK, MUON_MASS)
+    for p, vals in expected.items():
+        for g, e in zip(got[p], vals):
+            if e == 0.0:
+                assert g == 0.0
+            else:
+                assert float(f"{g:.4e}") == e, (p, g, e)
+
+
+def test_quadrature9_exact_on_monomials():
+    # nodes on [-1, 1] used with bounds (0, 1): one interval of width 1, so the
+    # rule integrates over [-1, 1] (weights sum to 2); exact to degree 17
+    for k in range(18):
+        exact = 0.0 if k % 2 else 2.0 / (k + 1)
+        assert abs(oracle.quadrature_monomial(9, k, 0.0, 1.0) - exact) < 2e-15, k
+    x, w = np.polynomial.legendre.leggauss(9)
+    for k in (0, 4, 8, 12, 16):
+        assert abs(oracle.quadrature_monomial(9, k, 0.0, 1.0) - np.sum(w * x ** k)) < 1e-15
+
+
+def test_quadrature8_truncated_literals():
+    # 8-decimal literals on [0, 1]: within 1e-8 of exact but not exact for k >= 2
+    x, w = np.polynomial.legendre.leggauss(8)
+    errs = []
+    for k in range(16):
+        got = oracle.quadrature_monomial(8, k, 0.0, 1.0)
+        assert abs(got - 1.0 / (k + 1)) < 1e-8, k
+        errs.append(abs(got - 0.5 * np.sum(w * ((x + 1) / 2) ** k)))
+    assert abs(oracle.quadrature_monomial(8, 0, 0.0, 1.0) - 1.0) < 1e-15
+    assert max(errs) > 1e-11  # the literals, not exact Gauss-Legendre nodes
+
+
+def test_quadrature6_subintervals():
+    # min_points = 180 -> 30 subintervals of [a, b]; weights sum to 1 + 2e-8
+    s = oracle.quadrature_monomial(6, 0, 0.0, 1.0, 1)
+    assert abs(s - (1.0 + 2e-8)) < 1e-15
+    a, b = np.log(1e-3), np.log(10.0)
+    got = oracle.quadrature_monomial(6, 3, a, b, 180)
+    assert abs(got / ((b ** 4 - a ** 4) / 4) - (1 + 2e-8)) < 1e-9
+
+
+@pytest.mark.skipif(oracle.load_ref() is None, reason="reference sources not present to build oracle/_ref")
+def test_quadrature_bitwise_equals_reference():
+    ref = oracle.load_ref()
+    for rule in (6, 8, 9):
+        for k in range(12):
+            for mp in (1, 7, 180):
+                a = oracle.quadrature_monomial(rule, k, -0.3, 2.1, mp)
+                b = ref.ref_quadrature_monomial(rule, k, -0.3, 2.1, mp)
+                assert a == b, (rule, k, mp)
+
+
+def test_bremsstrahlung_small_nu_limit():
+    # delta -> 0: Phi_n -> ln(B_n m / (D_n m_e)), Phi_e -> ln(B_e m / m_e) and
+    # dcs -> alpha r_e^2 (m_e/m)^2 Z (Z Phi_n + Phi_e) 4/(3 nu) * 1e3 N_A / A
+    el, m, me = STANDARD_ROCK, MUON_MASS, ELECTRON_MASS
+    Z, A = el.Z, el.A
+    Bn, Be, Dn = 182.7 * Z ** (-1 / 3), 1429.0 * Z ** (-2 / 3), 1.54 * A ** 0.27
+    K = np.array([1e3, 1e4, 1e5])
+    nu = 1e-9
+    E = K + m
+    q = nu * E
+    lim = (7.297182e-7 * (5.63588e-13 * me / m) ** 2 * Z * (Z * np.log(Bn * m / (Dn * me)) + np.log(Be * m / me))
+           * (4 / 3) / nu * 1e3 * AVOGADRO_NUMBER / A)
+    got = oracle.dcs_map(0, K, q, el, m)
+    assert np.allclose(got / lim, 1.0, rtol=1e-6, atol=0)
+
+
+def test_ionisation_close_collision_limit():
+    # K below the radiative-correction switch and q << Wmax:
+    # cs -> 1.535336e-5 E Z / A * E^2 / (P^2 q^2)
+    el, m = STANDARD_ROCK, MUON_MASS
+    K = np.array([0.5, 1.0, 2.0])
+    E = K + m
+    P2 = K * (K + 2 * m)
+    q = 1e-7 * np.ones(3)  # just above 0.62 I; the next term is -q / Wmax ~ 1e-6
+    lim = 1.535336e-5 * E * el.Z / el.A * (E * E / P2) / q ** 2
+    got = oracle.dcs_map(3, K, q, el, m)
+    assert np.allclose(got / lim, 1.0, rtol=1e-5, atol=0)
+
+
+def test_nonnegative_and_zero_masks():
+    el, m, me = STANDARD_ROCK, MUON_MASS, ELECTRON_MASS
+    K, q = workload.pairs(1 << 16, seed=21)
+    out = oracle.dcs_map_all(K, q, el, m)
+    assert np.all(np.isfinite(out)) and np.all(out >= 0)
+    assert np.all(out[0] > 0)  # bremsstrahlung has no kinematic exit
+    E = K + m
+    # pair production exits (dcs.hh:151-156, 172-175)
+    cut = K + m * (1 - 0.75 * 1.6487212707 * el.Z ** (1 / 3))
+    g = 1 + K / m
+    x0 = 4 * me / q
+    x1 = 6 / (g * (g - q / m))
+    with np.errstate(invalid="ignore"):
+        am = (x0 + 2 * (1 - x0) * x1) / (1 + (1 - x1) * np.sqrt(1 - x0))
+    pair_zero = (q <= 4 * me) | (q >= cut) | ~((am > 0) & (am < 1))
+    _check_exits(out[1], pair_zero)
+    # photonuclear exits (dcs.hh:357-381)
+    mpi, M = 0.134977, 0.931494
+    y = q / E
+    q2min = m * m * y * y / (1 - y)
+    q2max = 2 * M * (q - mpi) - mpi * mpi
+    photo_zero = (q < 1) | (q < 2e-3 * K) | (q >= E - m) | (q <= mpi * (1 + 0.5 * mpi / M)) | (q2max < q2min)
+    _check_exits(out[2], photo_zero)
+    # ionisation exits (dcs.hh:417-424)
+    P2 = K * (K + 2 * m)
+    wmax = 2 * me * P2 / (m * m + me * (me + 2 * E))
+    ion_zero = (wmax < 0.05 * K) | (q > wmax) | (q <= 0.62 * el.I)
+    assert np.array_equal(out[3] == 0, ion_zero)  # no clamp: zero exactly at the exits
+
+
+def _check_exits(dcs, exit_mask):
+    # an exit gives exactly 0; inside the kinematic range the result may still
+    # clamp to 0 (every node's screening term clamped), but only rarely
+    assert np.all(dcs[exit_mask] == 0)
+    assert np.mean(dcs[~exit_mask] == 0) < 1e-3
+
+
+def test_opcount_counts_per_pair_work():
+    # one bremsstrahlung pair below qe_max: two logs, no per-element pow
+    c = oracle.opcount(0, np.array([10.0]), np.array([0.1]), STANDARD_ROCK, MUON_MASS)
+    assert c["log"] == 2 and c["pow"] == 0 and c["exp"] == 0
+    # pair production: one log for tmin, 7 logs + 1 exp per node, 8 nodes,
+    # and two for zeta (gamma = 1 + K/m > 35 here)
+    c = oracle.opcount(1, np.array([10.0]), np.array([1.0]), STANDARD_ROCK, MUON_MASS)
+    assert c["exp"] == 8 and c["log"] == 1 + 7 * 8 + 2
