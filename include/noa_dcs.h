@@ -20,7 +20,10 @@
 
 #include <stdint.h>
 
-#include <hip/hip_runtime_api.h>
+/* The HIP stream handle, declared exactly as hip_runtime_api.h declares it,
+ * so that this header needs no ROCm headers (a C11/C++ repeat of an
+ * identical typedef is allowed). */
+typedef struct ihipStream_t* hipStream_t;
 
 #ifdef __cplusplus
 extern "C" {
@@ -112,6 +115,63 @@ int noa_dcs_batch_launch(noa_dcs_batch* plan, hipStream_t stream);
 /* Chosen tile and workgroups per CU, and the number of work items. */
 int noa_dcs_batch_info(const noa_dcs_batch* plan, int32_t* tile, int32_t* waves_per_cu, int64_t* n_items);
 int noa_dcs_batch_destroy(noa_dcs_batch* plan);
+
+/* ---- learned surrogate of the four DCS processes ----
+ * No reference counterpart: the reference evaluates only the exact formulas.
+ * An MLP (log10 K, log10 nu, Z) -> 4 hidden layers of 128 with SiLU -> the
+ * standardised ln(A * DCS) of the 4 processes; hidden and output layers on
+ * bf16 MFMA with FP32 accumulation, the first layer in FP32. Parameters are
+ * a flat FP32 vector of noa_dcs_mlp_num_params() values (W[out][in]
+ * row-major per layer, then its bias; layers 1..5). X is [n x 3] FP32
+ * features, T and M are [n x 4] FP32 targets and 0/1 masks (device). */
+typedef struct noa_dcs_mlp noa_dcs_mlp;
+
+int64_t noa_dcs_mlp_num_params(void);
+/* max_rows bounds the training batch and the forward chunk (<= 2^20). */
+int noa_dcs_mlp_create(uint64_t seed, int32_t max_rows, noa_dcs_mlp** mlp);
+int noa_dcs_mlp_destroy(noa_dcs_mlp* mlp);
+int noa_dcs_mlp_get_params(const noa_dcs_mlp* mlp, float* host, int64_t n);
+int noa_dcs_mlp_set_params(noa_dcs_mlp* mlp, const float* host, int64_t n);
+/* Features of n pairs of one element: x = ((log10 K - 1.5) / 4.5,
+ * (log10(q/K) + 3) / 3, Z / 46 - 1). */
+int noa_dcs_mlp_features(const double* K, const double* q, int64_t n, int32_t Z, float* X, hipStream_t stream);
+/* Layer-by-layer forward with the training kernels: Y [rows x 4]. */
+int noa_dcs_mlp_forward(noa_dcs_mlp* mlp, const float* X, int64_t rows, float* Y, hipStream_t stream);
+/* One forward + backward of the masked mean squared error on rows <=
+ * max_rows; loss and the gradient (num_params floats) to host memory. */
+int noa_dcs_mlp_forward_backward(noa_dcs_mlp* mlp, const float* X, const float* T, const float* M, int32_t rows,
+                                 double* loss, float* grads_host, hipStream_t stream);
+/* Adam over minibatches of a device dataset of n rows for `epochs` epochs,
+ * cosine learning rate lr_max -> lr_min; loss_per_epoch: host, may be NULL. */
+int noa_dcs_mlp_train(noa_dcs_mlp* mlp, const float* X, const float* T, const float* M, int64_t n, int32_t batch,
+                      int32_t epochs, float lr_max, float lr_min, uint64_t seed, float* loss_per_epoch,
+                      hipStream_t stream);
+/* The fused inference kernel (all layers chained in registers): Y [n x 4]. */
+int noa_dcs_mlp_infer_fused(noa_dcs_mlp* mlp, const float* X, int64_t n, float* Y, hipStream_t stream);
+
+typedef struct noa_dcs_surrogate_cfg {
+    float out_mean[4];      /* de-standardisation of the outputs: */
+    float out_std[4];       /*   ln(A * DCS_p) = out_mean[p] + out_std[p] * y_p */
+    const uint8_t* flag_map; /* device [n_elements][bins][bins] over (log10 K in [-3,6], log10 nu in [-6,0]);
+                                non-zero = evaluate that cell exactly; may be NULL */
+    int32_t bins;
+    int32_t recheck_per_mille; /* seeded sample evaluated exactly and compared (10 = 1%) */
+    uint64_t seed;
+} noa_dcs_surrogate_cfg;
+
+/* Surrogate DCS of the queries of an element batch (layout as in
+ * noa_dcs_batch_create), FP32 outputs. Queries outside the training domain,
+ * in flagged cells or in the recheck sample are evaluated by the exact FP64
+ * physics instead. fallback_idx (int64) and fallback_el (uint32) are device
+ * workspaces of offsets[n_elements] - offsets[0] entries; counters is a
+ * device array of 4 uint32: on completion counters[1] = number of queries
+ * evaluated exactly, counters[2] = bits of the largest relative error of the
+ * surrogate over the recheck sample (FP32). Asynchronous on `stream`; calls
+ * on one network are ordered by the stream. */
+int noa_dcs_surrogate_eval(noa_dcs_mlp* mlp, const noa_dcs_element* elements, const int64_t* offsets,
+                           int32_t n_elements, double mass, const double* K, const double* q,
+                           const noa_dcs_outputs_f32* out, const noa_dcs_surrogate_cfg* cfg, int64_t* fallback_idx,
+                           uint32_t* fallback_el, uint32_t* counters, hipStream_t stream);
 
 /* ---- synthetic workload (counter-based, seeded) ----
  * u(seed, stream, i) is a 53-bit uniform in [0, 1) from a SplitMix64 output

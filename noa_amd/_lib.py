@@ -51,6 +51,11 @@ class BatchConfig(ctypes.Structure):
                 ("tune_cache", ctypes.c_char_p)]
 
 
+class SurrogateConfig(ctypes.Structure):
+    _fields_ = [("out_mean", ctypes.c_float * 4), ("out_std", ctypes.c_float * 4), ("flag_map", ctypes.c_void_p),
+                ("bins", ctypes.c_int32), ("recheck_per_mille", ctypes.c_int32), ("seed", ctypes.c_uint64)]
+
+
 _P = ctypes.c_void_p
 _I64 = ctypes.c_int64
 _D = ctypes.c_double
@@ -71,6 +76,20 @@ SIGNATURES = {
     "noa_dcs_batch_info": (ctypes.c_int, [_P, ctypes.POINTER(_I32), ctypes.POINTER(_I32), ctypes.POINTER(_I64)]),
     "noa_dcs_batch_destroy": (ctypes.c_int, [_P]),
     "noa_dcs_fill_workload": (ctypes.c_int, [_P, _P, _I64, _U64, _U64, _D, _D, _D, _D, _P]),
+    "noa_dcs_mlp_num_params": (ctypes.c_int64, []),
+    "noa_dcs_mlp_create": (ctypes.c_int, [_U64, _I32, ctypes.POINTER(_P)]),
+    "noa_dcs_mlp_destroy": (ctypes.c_int, [_P]),
+    "noa_dcs_mlp_get_params": (ctypes.c_int, [_P, _P, _I64]),
+    "noa_dcs_mlp_set_params": (ctypes.c_int, [_P, _P, _I64]),
+    "noa_dcs_mlp_features": (ctypes.c_int, [_P, _P, _I64, _I32, _P, _P]),
+    "noa_dcs_mlp_forward": (ctypes.c_int, [_P, _P, _I64, _P, _P]),
+    "noa_dcs_mlp_forward_backward": (ctypes.c_int, [_P, _P, _P, _P, _I32, ctypes.POINTER(_D), _P, _P]),
+    "noa_dcs_mlp_train": (ctypes.c_int, [_P, _P, _P, _P, _I64, _I32, _I32, ctypes.c_float, ctypes.c_float, _U64,
+                                         _P, _P]),
+    "noa_dcs_mlp_infer_fused": (ctypes.c_int, [_P, _P, _I64, _P, _P]),
+    "noa_dcs_surrogate_eval": (ctypes.c_int, [_P, ctypes.POINTER(Element), ctypes.POINTER(_I64), _I32, _D, _P, _P,
+                                              ctypes.POINTER(OutputsF32), ctypes.POINTER(SurrogateConfig), _P, _P,
+                                              _P, _P]),
 }
 
 _lib = None

@@ -19,7 +19,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libnoa_dcs.so")
 BUILD_DIR = os.path.join(HERE, "_build")
 
-HIP_SOURCES = ["dcs_eval.hip", "dcs_batch.hip", "surrogate.hip"]
+HIP_SOURCES = ["dcs_eval.hip", "dcs_batch.hip", "surrogate_train.hip", "surrogate_infer.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [
@@ -60,6 +60,30 @@ def build_library(verbose: bool = False) -> str:
     return LIB
 
 
+def build_extension(verbose: bool = False) -> str:
+    """The torch extension noa_amd/_muon_dcs (csrc/muon_dcs_ext.cpp), a host
+    C++ module over include/noa/pms/dcs_hip.hh linked to libnoa_dcs.so."""
+    import sysconfig
+
+    import torch
+    from torch.utils import cpp_extension as ce
+    out = os.path.join(HERE, "_muon_dcs" + sysconfig.get_config_var("EXT_SUFFIX"))
+    tlib = os.path.join(os.path.dirname(torch.__file__), "lib")
+    cmd = ["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-w",
+           "-D__HIP_PLATFORM_AMD__", "-DUSE_ROCM", "-DTORCH_EXTENSION_NAME=_muon_dcs",
+           "-DTORCH_API_INCLUDE_EXTENSION_H", f"-D_GLIBCXX_USE_CXX11_ABI={int(torch._C._GLIBCXX_USE_CXX11_ABI)}",
+           "-I" + os.path.join(ROOT, "include"), "-I/opt/rocm/include",
+           *("-I" + p for p in ce.include_paths()), "-I" + sysconfig.get_paths()["include"],
+           os.path.join(CSRC, "muon_dcs_ext.cpp"), "-o", out + ".tmp",
+           "-L" + tlib, "-ltorch", "-ltorch_cpu", "-ltorch_python", "-lc10", "-lc10_hip",
+           "-L" + HERE, "-lnoa_dcs", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath," + tlib]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    _run(cmd)
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_oracle(with_reference: bool = True) -> None:
     oracle = os.path.join(ROOT, "oracle")
     _run(["make", "-C", oracle, "all", f"PY={sys.executable}"])
@@ -70,6 +94,7 @@ def build_oracle(with_reference: bool = True) -> None:
 def main() -> None:
     verbose = "-v" in sys.argv
     print("building", build_library(verbose), flush=True)
+    print("building", build_extension(verbose), flush=True)
     build_oracle()
     print("oracle built", flush=True)
 

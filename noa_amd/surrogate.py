@@ -1,0 +1,393 @@
+"""Learned surrogate of the four DCS processes, judged and backed by the exact
+FP64 kernel.
+
+Pipeline (all on the GPU):
+  1. dataset: seeded (K, q) pairs over an element batch, exact DCS from the
+     FP64 kernel; features (log10 K, log10 nu, Z) and targets ln(A * DCS)
+     standardised per process, masked where the DCS is zero;
+  2. training: hand-written bf16 MFMA forward/backward and Adam
+     (noa_dcs_mlp_train);
+  3. validation on held-out pairs against the FP64 kernel, per element and
+     per (log10 K, log10 nu) cell: cells whose worst relative error exceeds the
+     tolerance are flagged, and flagged queries are evaluated exactly;
+  4. inference: the fused register-resident MFMA kernel, exact FP64 fallback
+     for flagged / out-of-domain / seeded-recheck queries.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import time
+
+import numpy as np
+import torch
+
+from . import _lib, workload
+from .batch import BatchPlan
+from .physics import MUON_MASS
+
+NPROC = 4
+HERE = os.path.dirname(os.path.abspath(__file__))
+CHECKPOINT = os.path.join(HERE, "surrogate_48el.npz")
+MAP_BINS = 64
+
+
+def _p(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def _stream(dev):
+    return ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+
+
+class Network:
+    """The MLP 3 -> 128 x4 (SiLU) -> 4 with its device state."""
+
+    def __init__(self, seed: int = 0, max_rows: int = 8192, device=None):
+        self.device = device or torch.device("cuda", torch.cuda.current_device())
+        self.max_rows = max_rows
+        h = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            _lib.check(_lib.lib().noa_dcs_mlp_create(seed, max_rows, ctypes.byref(h)))
+        self.handle = h
+        self.nparam = _lib.lib().noa_dcs_mlp_num_params()
+
+    def get_params(self) -> np.ndarray:
+        out = np.zeros(self.nparam, dtype=np.float32)
+        _lib.check(_lib.lib().noa_dcs_mlp_get_params(self.handle, out.ctypes.data_as(ctypes.c_void_p), self.nparam))
+        return out
+
+    def set_params(self, p: np.ndarray) -> None:
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        _lib.check(_lib.lib().noa_dcs_mlp_set_params(self.handle, p.ctypes.data_as(ctypes.c_void_p), p.size))
+
+    def forward(self, X: torch.Tensor) -> torch.Tensor:
+        Y = torch.empty(X.shape[0], NPROC, dtype=torch.float32, device=X.device)
+        _lib.check(_lib.lib().noa_dcs_mlp_forward(self.handle, _p(X), X.shape[0], _p(Y), _stream(X.device)))
+        return Y
+
+    def forward_fused(self, X: torch.Tensor) -> torch.Tensor:
+        Y = torch.empty(X.shape[0], NPROC, dtype=torch.float32, device=X.device)
+        _lib.check(_lib.lib().noa_dcs_mlp_infer_fused(self.handle, _p(X), X.shape[0], _p(Y), _stream(X.device)))
+        return Y
+
+    def forward_backward(self, X, T, M) -> tuple[float, np.ndarray]:
+        loss = ctypes.c_double()
+        g = np.zeros(self.nparam, dtype=np.float32)
+        _lib.check(_lib.lib().noa_dcs_mlp_forward_backward(self.handle, _p(X), _p(T), _p(M), X.shape[0],
+                                                           ctypes.byref(loss), g.ctypes.data_as(ctypes.c_void_p),
+                                                           _stream(X.device)))
+        return loss.value, g
+
+    def train(self, X, T, M, epochs: int, batch: int = 8192, lr_max: float = 2e-3, lr_min: float = 2e-5,
+              seed: int = 0) -> np.ndarray:
+        losses = np.zeros(max(epochs, 1), dtype=np.float32)
+        _lib.check(_lib.lib().noa_dcs_mlp_train(self.handle, _p(X), _p(T), _p(M), X.shape[0], batch, epochs,
+                                                lr_max, lr_min, seed, losses.ctypes.data_as(ctypes.c_void_p),
+                                                _stream(X.device)))
+        return losses[:epochs]
+
+    def close(self) -> None:
+        if getattr(self, "handle", None) is not None:
+            _lib.lib().noa_dcs_mlp_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def features(K: torch.Tensor, q: torch.Tensor, offsets, elements) -> torch.Tensor:
+    """[n x 3] FP32 features, element by element (noa_dcs_mlp_features)."""
+    X = torch.empty(K.numel(), 3, dtype=torch.float32, device=K.device)
+    for e, el in enumerate(elements):
+        a, b = int(offsets[e]), int(offsets[e + 1])
+        if b > a:
+            _lib.check(_lib.lib().noa_dcs_mlp_features(_p(K[a:]), _p(q[a:]), b - a, int(el.Z), _p(X[a:]),
+                                                       _stream(K.device)))
+    return X
+
+
+def exact_dcs(K, q, offsets, elements, mass=MUON_MASS):
+    """Exact FP64 DCS of a batch (the persistent batched kernel)."""
+    plan = BatchPlan(elements, offsets, K, q, mass, precision="f64", tile=2048, waves_per_cu=4, use_graph=False)
+    plan.launch()
+    out = plan.outputs
+    plan.close()
+    return out
+
+
+def equal_offsets(n_elements: int, n_total: int) -> np.ndarray:
+    counts = np.full(n_elements, n_total // n_elements, dtype=np.int64)
+    counts[: n_total % n_elements] += 1
+    return np.concatenate([[0], np.cumsum(counts)])
+
+
+def make_pairs(offsets, seed: int, device):
+    n = int(offsets[-1])
+    K = torch.empty(n, dtype=torch.float64, device=device)
+    q = torch.empty_like(K)
+    workload.fill_device_batch(K, q, offsets, seed=seed)
+    return K, q
+
+
+def targets(dcs, elements, offsets, stats=None):
+    """ln(A * DCS) standardised per process, mask = DCS > 0."""
+    n = dcs[0].numel()
+    lnA = torch.empty(n, dtype=torch.float64, device=dcs[0].device)
+    for e, el in enumerate(elements):
+        lnA[int(offsets[e]):int(offsets[e + 1])] = float(np.log(el.A))
+    Y = torch.zeros(n, NPROC, dtype=torch.float64, device=lnA.device)
+    M = torch.zeros(n, NPROC, dtype=torch.float32, device=lnA.device)
+    for p in range(NPROC):
+        act = dcs[p] > 0
+        M[:, p] = act.float()
+        Y[act, p] = torch.log(dcs[p][act]) + lnA[act]
+    if stats is None:
+        mean = np.array([Y[M[:, p] > 0, p].mean().item() for p in range(NPROC)])
+        std = np.array([Y[M[:, p] > 0, p].std().item() for p in range(NPROC)])
+        stats = {"mean": mean, "std": std}
+    T = ((Y - torch.from_numpy(stats["mean"]).to(Y.device)) / torch.from_numpy(stats["std"]).to(Y.device)) * M
+    return T.float().contiguous(), M.contiguous(), stats
+
+
+class Surrogate:
+    """A trained network with its output statistics and validation error map."""
+
+    def __init__(self, net: Network, elements, stats: dict, flag_map: np.ndarray | None, tol: float,
+                 meta: dict | None = None):
+        self.net = net
+        self.elements = list(elements)
+        self.stats = stats
+        self.tol = tol
+        self.meta = meta or {}
+        self.flag_map_host = flag_map
+        self.flag_map = (torch.from_numpy(flag_map.astype(np.uint8)).to(net.device)
+                         if flag_map is not None else None)
+
+    def config(self, recheck_per_mille: int = 10, seed: int = 0, use_map: bool = True) -> _lib.SurrogateConfig:
+        c = _lib.SurrogateConfig()
+        for p in range(NPROC):
+            c.out_mean[p] = float(self.stats["mean"][p])
+            c.out_std[p] = float(self.stats["std"][p])
+        c.flag_map = _p(self.flag_map) if (use_map and self.flag_map is not None) else None
+        c.bins = MAP_BINS
+        c.recheck_per_mille = recheck_per_mille
+        c.seed = seed
+        return c
+
+    def evaluate(self, K, q, offsets, mass=MUON_MASS, recheck_per_mille: int = 10, seed: int = 0,
+                 use_map: bool = True, outputs=None, workspace=None):
+        """Surrogate DCS (FP32) with exact fallback; returns (outputs, n_exact, recheck_max_rel)."""
+        n = K.numel()
+        dev = K.device
+        if outputs is None:
+            outputs = tuple(torch.zeros(n, dtype=torch.float32, device=dev) for _ in range(NPROC))
+        if workspace is None:
+            workspace = self.workspace(n, dev)
+        fb_idx, fb_el, counters = workspace
+        els = (_lib.Element * len(self.elements))(*[_lib.Element(e.A, e.I, e.Z) for e in self.elements])
+        offs = (ctypes.c_int64 * len(offsets))(*[int(x) for x in offsets])
+        out = _lib.OutputsF32(*(_p(o) for o in outputs))
+        cfg = self.config(recheck_per_mille, seed, use_map)
+        with torch.cuda.device(dev):
+            _lib.check(_lib.lib().noa_dcs_surrogate_eval(self.net.handle, els, offs, len(self.elements), float(mass),
+                                                         _p(K), _p(q), ctypes.byref(out), ctypes.byref(cfg),
+                                                         _p(fb_idx), _p(fb_el), _p(counters), _stream(dev)))
+        return outputs, counters
+
+    @staticmethod
+    def workspace(n: int, dev):
+        return (torch.empty(max(n, 1), dtype=torch.int64, device=dev),
+                torch.empty(max(n, 1), dtype=torch.int32, device=dev),
+                torch.zeros(4, dtype=torch.int32, device=dev))
+
+    @staticmethod
+    def read_counters(counters) -> tuple[int, float]:
+        c = counters.cpu().numpy().astype(np.uint32)
+        return int(c[1]), float(np.frombuffer(c[2:3].tobytes(), dtype=np.float32)[0])
+
+    def save(self, path: str = CHECKPOINT) -> None:
+        np.savez_compressed(path, params=self.net.get_params(), mean=self.stats["mean"], std=self.stats["std"],
+                            flag_map=self.flag_map_host if self.flag_map_host is not None else np.zeros(0, np.uint8),
+                            elements=np.array([[e.A, e.I, e.Z] for e in self.elements]), tol=self.tol,
+                            meta=np.array([repr(self.meta)]))
+
+    @classmethod
+    def load(cls, path: str = CHECKPOINT, device=None) -> "Surrogate":
+        from .physics import AtomicElement
+        d = np.load(path, allow_pickle=False)
+        net = Network(device=device)
+        net.set_params(d["params"])
+        els = [AtomicElement(A=float(a), I=float(i), Z=int(z)) for a, i, z in d["elements"]]
+        fm = d["flag_map"]
+        return cls(net, els, {"mean": d["mean"], "std": d["std"]}, fm if fm.size else None, float(d["tol"]),
+                   {"saved": str(d["meta"][0])})
+
+
+def error_map(sur_out, exact, K, q, offsets, bins: int = MAP_BINS) -> torch.Tensor:
+    """Worst relative error per (element, log10 K bin, log10 nu bin) over the
+    four processes; a zero / non-zero mismatch counts as error 1."""
+    dev = K.device
+    n = K.numel()
+    el = torch.empty(n, dtype=torch.int64, device=dev)
+    for e in range(len(offsets) - 1):
+        el[int(offsets[e]):int(offsets[e + 1])] = e
+    nu = q / K
+    ik = ((torch.log10(K) + 3.0) / 9.0 * bins).long().clamp(0, bins - 1)
+    inu = ((torch.log10(nu) + 6.0) / 6.0 * bins).long().clamp(0, bins - 1)
+    cell = (el * bins + ik) * bins + inu
+    worst = torch.zeros(n, dtype=torch.float32, device=dev)
+    for p in range(NPROC):
+        s = sur_out[p].double()
+        e = exact[p]
+        rel = torch.where(e > 0, (s - e).abs() / e.clamp_min(1e-300), (s != 0).double())
+        worst = torch.maximum(worst, rel.float())
+    m = torch.zeros((len(offsets) - 1) * bins * bins, dtype=torch.float32, device=dev)
+    m.scatter_reduce_(0, cell, worst, reduce="amax")
+    return m.reshape(len(offsets) - 1, bins, bins)
+
+
+def max_rel_errors(out, exact) -> list[dict]:
+    res = []
+    for p in range(NPROC):
+        s = out[p].double()
+        e = exact[p]
+        nz = e > 0
+        rel = (s[nz] - e[nz]).abs() / e[nz]
+        res.append({"max_rel": float(rel.max().item()) if rel.numel() else 0.0,
+                    "p99_rel": float(torch.quantile(rel[:1 << 24].float(), 0.99).item()) if rel.numel() else 0.0,
+                    "zero_mismatch": int(((s != 0) != nz).sum().item())})
+    return res
+
+
+def train_surrogate(elements, n_train: int = 1 << 22, n_val: int = 1 << 24, epochs: int = 200, batch: int = 8192,
+                    tol: float = 1e-3, seed: int = 0, device=None, log=print) -> tuple[Surrogate, dict]:
+    dev = device or torch.device("cuda", 0)
+    t0 = time.time()
+    off_tr = equal_offsets(len(elements), n_train)
+    K, q = make_pairs(off_tr, seed=1000 + seed, device=dev)
+    ex = exact_dcs(K, q, off_tr, elements)
+    X = features(K, q, off_tr, elements)
+    T, M, stats = targets(ex, elements, off_tr)
+    del ex
+    log(f"dataset {n_train} pairs in {time.time() - t0:.1f}s; active fraction per process "
+        f"{[round(float(M[:, p].mean()), 3) for p in range(NPROC)]}")
+    net = Network(seed=seed, max_rows=batch, device=dev)
+    t1 = time.time()
+    losses = net.train(X, T, M, epochs=epochs, batch=batch, seed=seed)
+    torch.cuda.synchronize()
+    t_train = time.time() - t1
+    log(f"trained {epochs} epochs in {t_train:.1f}s; loss {losses[0]:.3e} -> {losses[-1]:.3e}")
+    del K, q, X, T, M
+    # validation on held-out pairs (disjoint seed): error map and flags
+    off_va = equal_offsets(len(elements), n_val)
+    Kv, qv = make_pairs(off_va, seed=2000 + seed, device=dev)
+    exv = exact_dcs(Kv, qv, off_va, elements)
+    sur = Surrogate(net, elements, stats, None, tol)
+    outv, cnt = sur.evaluate(Kv, qv, off_va, recheck_per_mille=0, use_map=False)
+    emap = error_map(outv, exv, Kv, qv, off_va)
+    flags = (emap > 0.5 * tol).cpu().numpy().astype(np.uint8)
+    raw = max_rel_errors(outv, exv)
+    sur.flag_map_host = flags
+    sur.flag_map = torch.from_numpy(flags).to(dev)
+    report = {"n_train": n_train, "n_val": n_val, "epochs": epochs, "batch": batch, "train_s": t_train,
+              "loss_first": float(losses[0]), "loss_last": float(losses[-1]), "tol": tol,
+              "val_raw_errors": raw, "flagged_cells": float(flags.mean()),
+              "median_cell_err": float(emap.median().item())}
+    sur.meta = report
+    log(f"validation raw max rel err {[round(r['max_rel'], 4) for r in raw]}, p99 "
+        f"{[round(r['p99_rel'], 5) for r in raw]}; flagged cells {flags.mean():.3f}")
+    return sur, report
+
+
+def smoke(dev) -> str:
+    """Tiny end-to-end pass: a few training steps, fused == layered forward,
+    and the surrogate evaluation with everything routed exactly."""
+    from .physics import STANDARD_ROCK
+    els = [STANDARD_ROCK]
+    off = np.array([0, 8192])
+    K, q = make_pairs(off, seed=7, device=dev)
+    ex = exact_dcs(K, q, off, els)
+    X = features(K, q, off, els)
+    T, M, stats = targets(ex, els, off)
+    net = Network(seed=1, max_rows=1024, device=dev)
+    losses = net.train(X, T, M, epochs=2, batch=1024)
+    d = (net.forward(X) - net.forward_fused(X)).abs().max().item()
+    assert d < 2e-2, d
+    sur = Surrogate(net, els, stats, np.ones((1, MAP_BINS, MAP_BINS), np.uint8), 1e-3)
+    out, cnt = sur.evaluate(K, q, off, recheck_per_mille=0)
+    nfb, _ = Surrogate.read_counters(cnt)
+    assert nfb == 8192
+    for p in range(NPROC):
+        assert torch.equal(out[p], ex[p].float()), p
+    net.close()
+    return f"surrogate smoke ok (loss {losses[0]:.3f} -> {losses[-1]:.3f}, fused-vs-layered {d:.1e})"
+
+
+def bench_entry(K, q, ref, steps, warmup, stream, wall_f64) -> dict:
+    """Config-4 style measurement: the committed checkpoint on 2^28 queries
+    over its 48 elements, 1% seeded recheck, flagged cells exact; errors
+    against the FP64 kernel over every query."""
+    if not os.path.exists(CHECKPOINT):
+        raise FileNotFoundError(CHECKPOINT)
+    dev = K.device
+    sur = Surrogate.load(CHECKPOINT, device=dev)
+    n = 1 << 28
+    off = equal_offsets(len(sur.elements), n)
+    Kq, qq = make_pairs(off, seed=4, device=dev)
+    outs = tuple(torch.empty(n, dtype=torch.float32, device=dev) for _ in range(NPROC))
+    ws = Surrogate.workspace(n, dev)
+    for _ in range(max(1, warmup // 2)):
+        sur.evaluate(Kq, qq, off, outputs=outs, workspace=ws)
+    torch.cuda.synchronize()
+    reps = max(1, min(steps, 5))
+    t = time.perf_counter()
+    for _ in range(reps):
+        sur.evaluate(Kq, qq, off, outputs=outs, workspace=ws)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t) / reps
+    n_exact, recheck = Surrogate.read_counters(ws[2])
+    # the exact FP64 kernel on the same queries, for the speedup and the errors
+    plan = BatchPlan(sur.elements, off, Kq, qq, MUON_MASS, precision="f64", tile=2048, waves_per_cu=4)
+    plan.launch()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        plan.launch()
+    torch.cuda.synchronize()
+    dt_exact = (time.perf_counter() - t) / reps
+    errs = max_rel_errors(outs, plan.outputs)
+    plan.close()
+    return {"evals_per_s": n / dt, "n_queries": n, "exact_fraction": n_exact / n,
+            "recheck_max_rel": recheck, "max_rel_err_vs_f64": errs,
+            "exact_f64_evals_per_s_same_queries": n / dt_exact, "speedup_vs_f64": dt_exact / dt,
+            "flagged_cells": float(sur.flag_map_host.mean()) if sur.flag_map_host is not None else 0.0,
+            "tolerance": sur.tol}
+
+
+def main() -> None:
+    """python -m noa_amd.surrogate [--epochs E] [--out PATH]: trains on the
+    48-element batch (2^22 samples), validates on 2^24 held-out pairs and
+    writes the checkpoint with its error map."""
+    import argparse
+    import json
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--epochs", type=int, default=200)
+    ap.add_argument("--batch", type=int, default=8192)
+    ap.add_argument("--n-train", type=int, default=1 << 22)
+    ap.add_argument("--n-val", type=int, default=1 << 24)
+    ap.add_argument("--tol", type=float, default=1e-3)
+    ap.add_argument("--out", default=CHECKPOINT)
+    a = ap.parse_args()
+    elements, _ = workload.element_batch(48, seed=0)
+    sur, report = train_surrogate(elements, n_train=a.n_train, n_val=a.n_val, epochs=a.epochs, batch=a.batch,
+                                  tol=a.tol, log=lambda s: print(s, flush=True))
+    sur.save(a.out)
+    print(json.dumps(report), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -15,7 +15,7 @@ HEADER = os.path.join(ROOT, "include", "noa_dcs.h")
 
 def header_functions() -> set:
     text = open(HEADER).read()
-    return set(re.findall(r"^(?:int|const char\*)\s+(noa_dcs_\w+)\s*\(", text, re.M))
+    return set(re.findall(r"^(?:int|int64_t|const char\*)\s+(noa_dcs_\w+)\s*\(", text, re.M))
 
 
 def test_header_declares_the_bound_signatures():

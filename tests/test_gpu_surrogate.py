@@ -1,0 +1,144 @@
+"""The surrogate's hand-written kernels on the GPU: gradients against the CPU
+FP64 network, the fused inference kernel against the layered training
+forward, training progress, exact fallback routing, and what the trained
+network gains over the method without it (the exact kernel alone)."""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from noa_amd import surrogate as S
+from noa_amd.physics import MUON_MASS, STANDARD_ROCK
+from tests import mlp_ref
+
+pytestmark = pytest.mark.gpu
+
+
+def _batch(n, seed):
+    rng = np.random.default_rng(seed)
+    X = rng.uniform(-1, 1, (n, 3)).astype(np.float32)
+    T = rng.normal(0, 1, (n, 4)).astype(np.float32)
+    M = (rng.uniform(0, 1, (n, 4)) < 0.7).astype(np.float32)
+    return X, T, M
+
+
+def test_gradients_match_cpu_fp64_network(device):
+    net = S.Network(seed=3, max_rows=1024, device=device)
+    X, T, M = _batch(1000, 0)
+    td = lambda a: torch.from_numpy(a).to(device)  # noqa: E731
+    loss, g = net.forward_backward(td(X), td(T), td(M))
+    p = net.get_params()
+    # bf16-emulating FP64 reference: same rounding points, exact arithmetic otherwise
+    ref_loss, ref_g = mlp_ref.loss_and_grad(p, X, T, M, round_bf16=True)
+    assert abs(loss / ref_loss - 1) < 1e-4, (loss, ref_loss)
+    for name, (o, s) in mlp_ref.OFF.items():
+        a, b = g[o:o + s].astype(np.float64), ref_g[o:o + s]
+        rel = np.linalg.norm(a - b) / np.linalg.norm(b)
+        print(name, "relative L2 gradient error", rel, flush=True)
+        assert rel < 2e-3, (name, rel)
+    # the exact (no bf16) network: the gradient direction is the same
+    ex_loss, ex_g = mlp_ref.loss_and_grad(p, X, T, M, round_bf16=False)
+    assert abs(loss / ex_loss - 1) < 1e-2
+    cos = np.dot(g, ex_g) / (np.linalg.norm(g) * np.linalg.norm(ex_g))
+    assert cos > 0.999, cos
+    # a finite-difference probe of the exact network agrees with its own analytic gradient
+    k = mlp_ref.OFF["W3"][0] + 777
+    h = 1e-4
+    pp, pm = p.astype(np.float64).copy(), p.astype(np.float64).copy()
+    pp[k] += h
+    pm[k] -= h
+    fd = (mlp_ref.loss_and_grad(pp, X, T, M)[0] - mlp_ref.loss_and_grad(pm, X, T, M)[0]) / (2 * h)
+    assert abs(fd - ex_g[k]) < 1e-6 + 1e-4 * abs(ex_g[k])
+    net.close()
+
+
+def test_forward_matches_cpu_and_fused_matches_layered(device):
+    net = S.Network(seed=5, max_rows=4096, device=device)
+    X, _, _ = _batch(6000, 1)  # > max_rows: the layered forward chunks
+    Xd = torch.from_numpy(X).to(device)
+    Yl = net.forward(Xd).cpu().numpy()
+    Yf = net.forward_fused(Xd).cpu().numpy()
+    Yr, _, _ = mlp_ref.forward(net.get_params(), X, round_bf16=True)
+    # equal up to FP32 accumulation order, except where an activation sits on a
+    # bf16 rounding boundary and rounds the other way (a 2^-8 step, rare)
+    d = np.abs(Yl - Yr)
+    scale = np.max(np.abs(Yr))
+    assert np.median(d) < 1e-6 * scale and np.mean(d > 1e-5 * scale) < 0.01 and d.max() < 1e-2 * scale
+    d = np.abs(Yf - Yl)
+    print("fused vs layered: max", d.max(), "mean", d.mean(), flush=True)
+    assert d.max() < 2e-2 and d.mean() < 1e-3
+    net.close()
+
+
+def test_training_reduces_loss(device):
+    els = [STANDARD_ROCK]
+    off = np.array([0, 1 << 16])
+    K, q = S.make_pairs(off, seed=3, device=device)
+    ex = S.exact_dcs(K, q, off, els)
+    X = S.features(K, q, off, els)
+    T, M, stats = S.targets(ex, els, off)
+    net = S.Network(seed=2, max_rows=2048, device=device)
+    losses = net.train(X, T, M, epochs=30, batch=2048)
+    print("losses", losses[[0, 9, 19, 29]], flush=True)
+    assert losses[-1] < 0.05 * losses[0]
+    net.close()
+
+
+def test_flagged_and_rechecked_queries_are_exact(device):
+    els = [STANDARD_ROCK]
+    off = np.array([0, 20000])
+    K, q = S.make_pairs(off, seed=5, device=device)
+    ex = S.exact_dcs(K, q, off, els)
+    net = S.Network(seed=4, device=device)
+    stats = {"mean": np.zeros(4), "std": np.ones(4)}
+    sur = S.Surrogate(net, els, stats, np.ones((1, S.MAP_BINS, S.MAP_BINS), np.uint8), 1e-3)
+    out, cnt = sur.evaluate(K, q, off, recheck_per_mille=0)
+    assert S.Surrogate.read_counters(cnt)[0] == 20000
+    for p in range(4):
+        assert torch.equal(out[p], ex[p].float())
+    sur.flag_map = None
+    out, cnt = sur.evaluate(K, q, off, recheck_per_mille=1000)
+    n_exact, recheck = S.Surrogate.read_counters(cnt)
+    assert n_exact == 20000 and recheck > 0  # an untrained network is far off
+    for p in range(4):
+        assert torch.equal(out[p], ex[p].float())
+    # no routing: the zero pattern follows the exact kinematic exits; it can
+    # differ only where an exact result clamps to zero inside the kinematic
+    # range (every node's screening term clamped), which is rare
+    out, cnt = sur.evaluate(K, q, off, recheck_per_mille=0)
+    assert S.Surrogate.read_counters(cnt)[0] == 0
+    for p in range(4):
+        mismatch = ((out[p] == 0) != (ex[p] == 0)).double().mean().item()
+        assert mismatch < 2e-3, (p, mismatch)
+    # out-of-domain queries go to the exact path
+    K2 = K.clone()
+    K2[:100] = 5e6
+    out, cnt = sur.evaluate(K2, q, off, recheck_per_mille=0)
+    assert S.Surrogate.read_counters(cnt)[0] == 100
+    net.close()
+
+
+@pytest.mark.skipif(not os.path.exists(S.CHECKPOINT), reason="no trained checkpoint")
+def test_trained_network_gain(device):
+    sur = S.Surrogate.load(device=device)
+    off = S.equal_offsets(len(sur.elements), 1 << 22)
+    K, q = S.make_pairs(off, seed=77, device=device)
+    ex = S.exact_dcs(K, q, off, sur.elements)
+    # 1. raw network, no routing: the trained network against an untrained one
+    out, _ = sur.evaluate(K, q, off, recheck_per_mille=0, use_map=False)
+    trained = S.max_rel_errors(out, ex)
+    blank = S.Surrogate(S.Network(seed=9, device=device), sur.elements, sur.stats, None, sur.tol)
+    out0, _ = blank.evaluate(K, q, off, recheck_per_mille=0, use_map=False)
+    untrained = S.max_rel_errors(out0, ex)
+    print("trained p99", [r["p99_rel"] for r in trained], "untrained p99", [r["p99_rel"] for r in untrained])
+    for a, b in zip(trained, untrained):
+        assert a["p99_rel"] < 0.05 * b["p99_rel"]
+    # 2. with the validation map: every query within the stated tolerance
+    out, cnt = sur.evaluate(K, q, off, recheck_per_mille=10)
+    n_exact, recheck = S.Surrogate.read_counters(cnt)
+    routed = S.max_rel_errors(out, ex)
+    print("routed", routed, "exact fraction", n_exact / K.numel(), "recheck", recheck, flush=True)
+    for r in routed:
+        assert r["max_rel"] <= sur.tol, r
+    assert n_exact < K.numel()
