@@ -85,6 +85,24 @@ int noa_dcs_eval_f64(const double* K, const double* q, int64_t n, const noa_dcs_
 int noa_dcs_eval_f32mixed(const double* K, const double* q, int64_t n, const noa_dcs_element* element,
                           double mass, const noa_dcs_outputs_f32* out, hipStream_t stream);
 
+/*
+ * DEL and CEL recoil integrals of the four processes for n kinetic energies.
+ * Replaces dcs::vmap_integral(dcs::recoil_integral(f, dcs::del_integrand))
+ * and the cel_integrand form (src/noa/pms/dcs.hh:89-130), including the
+ * analytic ionisation branch below K = (m - m_e)^2 / (2 m_e)
+ * (dcs.hh:444-497, 955-1001):
+ *   del[p][i] = (1/E) int_{K xlow}^{K} dcs_p(K, q) dq,   cel[p][i] = (1/E) int dcs_p(K, q) q dq,
+ * integrated over t = ln q (integrand dcs q, resp. dcs q^2) with the 6-node
+ * rule on ceil(min_points/6) subintervals.
+ * Any output pointer may be NULL (del or cel themselves too). workspace is
+ * a device buffer of at least noa_dcs_recoil_workspace_bytes(n, min_points).
+ */
+int64_t noa_dcs_recoil_workspace_bytes(int64_t n, int32_t min_points);
+int noa_dcs_recoil_integrals_f64(const double* K, int64_t n, double xlow, const noa_dcs_element* element,
+                                 double mass, int32_t min_points, const noa_dcs_outputs_f64* del,
+                                 const noa_dcs_outputs_f64* cel, void* workspace, int64_t workspace_bytes,
+                                 hipStream_t stream);
+
 /* ---- batched uneven elements: one persistent launch, captured in a hipGraph ----
  * Element e owns pairs [offsets[e], offsets[e+1]) of the concatenated K, q
  * and output arrays. The reference has no batched form: a caller loops

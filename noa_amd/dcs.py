@@ -192,6 +192,62 @@ hip = _hip_namespace()
 cuda = hip  # the reference's dcs::cuda::{vmap,map}_bremsstrahlung names (dcs.hh:1004-1019)
 
 
+class _Integrand:
+    def __init__(self, name: str, cel: bool):
+        self.name = name
+        self.cel = cel
+
+    def __repr__(self) -> str:
+        return f"dcs.{self.name}"
+
+
+del_integrand = _Integrand("del_integrand", False)   # dcs.hh:107-109: dcs * q
+cel_integrand = _Integrand("cel_integrand", True)    # dcs.hh:111-113: dcs * q^2
+
+
+def recoil_integrals(kinetic_energies: torch.Tensor, xlow: float, element: AtomicElement, mass: float,
+                     min_points: int = 180, dels=None, cels=None) -> None:
+    """DEL / CEL recoil integrals of the four processes in one pass (noa_dcs_recoil_integrals_f64):
+    dels and cels are 4-tuples of float64 tensors like K, or None entries."""
+    K = kinetic_energies
+    _check_inputs(K, K)
+    dels = tuple(dels) if dels is not None else (None,) * 4
+    cels = tuple(cels) if cels is not None else (None,) * 4
+    for r in dels + cels:
+        _check_output(r, K, torch.float64)
+    nbytes = _lib.lib().noa_dcs_recoil_workspace_bytes(K.numel(), int(min_points))
+    ws = torch.empty(max(nbytes, 8), dtype=torch.uint8, device=K.device)
+    d = _lib.OutputsF64(*(_ptr(r) for r in dels))
+    c = _lib.OutputsF64(*(_ptr(r) for r in cels))
+    el = _element(element)
+    with torch.cuda.device(K.device):
+        _lib.check(_lib.lib().noa_dcs_recoil_integrals_f64(_ptr(K), K.numel(), float(xlow), ctypes.byref(el),
+                                                           float(mass), int(min_points), ctypes.byref(d),
+                                                           ctypes.byref(c), _ptr(ws), nbytes, _stream(K)))
+
+
+def recoil_integral(dcs_func: Process, integrand: _Integrand):
+    """dcs.hh:89-105 (and the ionisation specialisations, dcs.hh:955-1001):
+    an integral object for vmap_integral."""
+    if not isinstance(integrand, _Integrand):
+        raise TypeError("integrand must be dcs.del_integrand or dcs.cel_integrand")
+    return (_process(dcs_func), integrand)
+
+
+def vmap_integral(cs_integral):
+    """dcs.hh:115-130: f(result, K, xlow, element, mass, min_points) in place."""
+    p, integrand = cs_integral
+
+    def run(result, kinetic_energies, xlow, element, mass, min_points):
+        outs = [None] * 4
+        outs[p.id] = result
+        if integrand.cel:
+            recoil_integrals(kinetic_energies, xlow, element, mass, min_points, cels=outs)
+        else:
+            recoil_integrals(kinetic_energies, xlow, element, mass, min_points, dels=outs)
+    return run
+
+
 def relative_error(computed: torch.Tensor, expected: torch.Tensor) -> torch.Tensor:
     """common.hh:228-236: mean(|c - e| / (c + numeric_limits<T>::min()))."""
     tiny = torch.finfo(computed.dtype).tiny

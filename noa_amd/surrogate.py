@@ -277,7 +277,17 @@ def train_surrogate(elements, n_train: int = 1 << 22, n_val: int = 1 << 24, epoc
         f"{[round(float(M[:, p].mean()), 3) for p in range(NPROC)]}")
     net = Network(seed=seed, max_rows=batch, device=dev)
     t1 = time.time()
-    losses = net.train(X, T, M, epochs=epochs, batch=batch, seed=seed)
+    # chunks of epochs, each with its piece of one cosine schedule, so that
+    # progress is reported while training runs
+    lr_max, lr_min, chunk = 2e-3, 2e-5, 10
+    cos = lambda f: lr_min + (lr_max - lr_min) * 0.5 * (1 + np.cos(np.pi * f))  # noqa: E731
+    losses = []
+    for c0 in range(0, epochs, chunk):
+        c1 = min(epochs, c0 + chunk)
+        losses += list(net.train(X, T, M, epochs=c1 - c0, batch=batch, lr_max=cos(c0 / epochs),
+                                 lr_min=cos(c1 / epochs), seed=seed + c0))
+        log(f"epoch {c1}/{epochs}: loss {losses[-1]:.3e} ({time.time() - t1:.0f}s)")
+    losses = np.array(losses)
     torch.cuda.synchronize()
     t_train = time.time() - t1
     log(f"trained {epochs} epochs in {t_train:.1f}s; loss {losses[0]:.3e} -> {losses[-1]:.3e}")

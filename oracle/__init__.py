@@ -108,6 +108,34 @@ def ref_map(process: int, K, q, element, mass: float, parallel: bool = False) ->
     return out
 
 
+_RECOIL_ARGS = [ctypes.c_int, ctypes.c_int, _D, ctypes.c_int64, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                ctypes.c_int32, ctypes.c_double]
+
+
+def recoil(process: int, cel: int, K, xlow: float, element, mass: float, min_points: int = 180,
+           threads: int = 1) -> np.ndarray:
+    """DEL (cel=0) or CEL (cel=1) recoil integral of one process at each K."""
+    lib = load()
+    lib.oracle_recoil_integral.argtypes = _RECOIL_ARGS + [ctypes.c_uint32, _D, ctypes.c_int]
+    K = np.ascontiguousarray(K, dtype=np.float64)
+    out = np.zeros_like(K)
+    assert lib.oracle_recoil_integral(process, cel, _p(K), K.size, xlow, element.A, element.I, element.Z, mass,
+                                      min_points, _p(out), threads) == 0
+    return out
+
+
+def ref_recoil(process: int, cel: int, K, xlow: float, element, mass: float, min_points: int = 180) -> np.ndarray:
+    ref = load_ref()
+    if ref is None:
+        raise FileNotFoundError(REF_PATH)
+    ref.ref_recoil_integral.argtypes = _RECOIL_ARGS + [ctypes.c_int32, _D]
+    K = np.ascontiguousarray(K, dtype=np.float64)
+    out = np.zeros_like(K)
+    assert ref.ref_recoil_integral(process, cel, _p(K), K.size, xlow, element.A, element.I, element.Z, mass,
+                                   min_points, _p(out)) == 0
+    return out
+
+
 def quadrature_monomial(rule: int, k: int, lb: float, ub: float, min_points: int = 1) -> float:
     return load().oracle_quadrature_monomial(rule, k, lb, ub, min_points)
 

@@ -101,4 +101,22 @@ int oracle_opcount(int process, const double* K, const double* q, int64_t n, dou
 
 int oracle_max_threads(void) { return omp_get_max_threads(); }
 
+// result[i] = recoil integral of `process` at K[i] (dcs.hh:115-130 vmap_integral
+// over dcs.hh:89-105 / 955-1001); cel = 0 for DEL (dcs * q), 1 for CEL (dcs * q^2).
+int oracle_recoil_integral(int process, int cel, const double* K, int64_t n, double xlow, double A, double I,
+                           int32_t Z, double mass, uint32_t min_points, double* result, int threads) {
+    if (process < 0 || process > 3) return -1;
+    const Element<double> el{A, I, Z};
+    if (threads == 1) {
+        for (int64_t i = 0; i < n; i++)
+            result[i] = noa_oracle::process_recoil_integral<double>(process, cel != 0, K[i], xlow, el, mass, min_points);
+    } else {
+        if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for default(none) shared(n, K, el, mass, result, process, cel, xlow, min_points) schedule(dynamic, 16)
+        for (int64_t i = 0; i < n; i++)
+            result[i] = noa_oracle::process_recoil_integral<double>(process, cel != 0, K[i], xlow, el, mass, min_points);
+    }
+    return 0;
+}
+
 }  // extern "C"

@@ -360,4 +360,64 @@ inline T ionisation(const T K, const T q, const Element<T>& el, const T mass) {
     return cs * (T(1.) + Delta);
 }
 
+// ---- recoil integrals (dcs.hh:89-130, 444-497, 955-1001) ----
+
+// dcs.hh:89-105 with del_integrand (dcs * q, :107-109) or cel_integrand
+// (dcs * q^2, :111-113): the 6-node rule over t = ln q in [ln(K xlow), ln K]
+// with ceil(min_points / 6) subintervals, divided by E = K + mass.
+template <class T, class F>
+inline T recoil_integral(const F& dcs, bool cel, const T K, const T xlow, const Element<T>& el, const T mass,
+                         uint32_t min_points) {
+    using std::log;
+    using std::exp;
+    auto f = [&](const T t) {
+        const T q = exp(t);
+        const T d = dcs(K, q, el, mass);
+        return cel ? d * q * q : d * q;
+    };
+    return gauss_legendre<T>(log(K * xlow), log(K), f, min_points, 6, GL6_X, GL6_W) / (K + mass);
+}
+
+// dcs.hh:469-496 with the close-interaction terms of :446-466.
+template <class T>
+inline T analytic_ionisation_recoil_integral(const T K, const T xlow, const Element<T>& el, const T mass,
+                                             bool cel) {
+    using std::log;
+    const T me = T(ELECTRON_MASS);
+    const T P2 = K * (K + T(2.) * mass);
+    const T E = K + mass;
+    const T Wmax = T(2.) * me * P2 / (mass * mass + me * (me + T(2.) * E));
+    if (Wmax < T(X_FRACTION) * K) return T(0.);
+    T Wmin = T(0.62) * el.I;
+    const T qlow = K * xlow;
+    if (qlow >= Wmin) Wmin = qlow;
+    if (Wmax <= Wmin) return T(0.);
+    const T a0 = T(0.5) / P2, a1 = T(-1.) / Wmax, a2 = E * E / P2;
+    T term;
+    if (cel)
+        term = T(0.5) * a0 * (Wmax * Wmax - Wmin * Wmin) + a1 * (Wmax - Wmin) + a2 * log(Wmax / Wmin);
+    else
+        term = a0 * (Wmax - Wmin) + a1 * log(Wmax / Wmin) + a2 * (T(1.) / Wmin - T(1.) / Wmax);
+    return T(1.535336E-05) * T(double(el.Z)) / el.A * term;
+}
+
+// The recoil integral of one process as the reference dispatches it: the
+// ionisation specialisations (dcs.hh:955-1001) take the analytic form for
+// K <= (m - m_e)^2 / (2 m_e) and the numerical one above.
+template <class T>
+inline T process_recoil_integral(int process, bool cel, const T K, const T xlow, const Element<T>& el,
+                                 const T mass, uint32_t min_points) {
+    switch (process) {
+        case 0: return recoil_integral<T>(bremsstrahlung<T>, cel, K, xlow, el, mass, min_points);
+        case 1: return recoil_integral<T>(pair_production<T>, cel, K, xlow, el, mass, min_points);
+        case 2: return recoil_integral<T>(photonuclear<T>, cel, K, xlow, el, mass, min_points);
+        default: {
+            const T m1 = mass - T(ELECTRON_MASS);
+            if (K <= T(0.5) * m1 * m1 / T(ELECTRON_MASS))
+                return analytic_ionisation_recoil_integral<T>(K, xlow, el, mass, cel);
+            return recoil_integral<T>(ionisation<T>, cel, K, xlow, el, mass, min_points);
+        }
+    }
+}
+
 }  // namespace noa_oracle

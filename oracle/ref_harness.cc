@@ -49,6 +49,29 @@ int ref_dcs_map(int process, const double* K, const double* q, int64_t n, double
     return 0;
 }
 
+// dcs::vmap_integral(dcs::recoil_integral(f, integrand)) (dcs.hh:89-130,
+// with the ionisation specialisations of dcs.hh:955-1001); cel = 0 for
+// del_integrand, 1 for cel_integrand.
+int ref_recoil_integral(int process, int cel, const double* K, int64_t n, double xlow, double A, double I,
+                        int32_t Z, double mass, int32_t min_points, double* result) {
+    const auto k = wrap(K, n), r = wrap(result, n);
+    const noa::pms::AtomicElement el{A, I, Z};
+    auto go = [&](const auto& f) {
+        if (cel)
+            dcs::vmap_integral(dcs::recoil_integral(f, dcs::cel_integrand))(r, k, xlow, el, mass, min_points);
+        else
+            dcs::vmap_integral(dcs::recoil_integral(f, dcs::del_integrand))(r, k, xlow, el, mass, min_points);
+    };
+    switch (process) {
+        case 0: go(dcs::bremsstrahlung); break;
+        case 1: go(dcs::pair_production); break;
+        case 2: go(dcs::photonuclear); break;
+        case 3: go(dcs::ionisation); break;
+        default: return -1;
+    }
+    return 0;
+}
+
 // noa::utils::relative_error (common.hh:228-236).
 double ref_relative_error(const double* computed, const double* expected, int64_t n) {
     return noa::utils::relative_error(wrap(computed, n), wrap(expected, n)).item<double>();

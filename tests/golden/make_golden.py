@@ -91,6 +91,20 @@ def main() -> None:
     b = block(STANDARD_ROCK, TAU_MASS, seed=12)
     for k, v in b.items():
         arrays[f"rock_tau_{k}"] = v
+    # DEL / CEL recoil integrals (dcs::vmap_integral(dcs::recoil_integral(...)),
+    # dcs.hh:89-130, 955-1001): xlow = X_FRACTION, min_points 180 as in
+    # test/unit/test-dcs-calc.cc:22-132, plus 7 and 61; K over the whole
+    # range and around the ionisation analytic switch
+    m1 = MUON_MASS - ELECTRON_MASS
+    kthr = 0.5 * m1 * m1 / ELECTRON_MASS
+    K = np.concatenate([10.0 ** np.linspace(-3, 6, 97), kthr * (1 + np.array([-1e-9, 0.0, 1e-9]))])
+    for name in ("rock", "hydrogen"):
+        el = ELEMENTS[name]
+        for mp in (180, 7, 61):
+            out = np.stack([[oracle.ref_recoil(p, cel, K, 0.05, el, MUON_MASS, mp) for p in range(4)]
+                            for cel in (0, 1)])
+            arrays[f"recoil_{name}_mp{mp}"] = out
+    arrays["recoil_K"] = K
     np.savez_compressed(os.path.join(HERE, "dcs_golden.npz"), **arrays)
     print({k: v.shape for k, v in arrays.items()})
 

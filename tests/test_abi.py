@@ -105,3 +105,19 @@ def test_check_raises_with_message():
     assert lib.noa_dcs_batch_launch(None, None) == _lib.NOA_DCS_EINVAL
     with pytest.raises(_lib.NoaDcsError, match="plan is NULL"):
         _lib.check(_lib.NOA_DCS_EINVAL)
+
+
+def test_recoil_argument_errors():
+    lib = _lib.lib()
+    rock = _lib.Element(22.0, 0.1364e-6, 11)
+    o = _lib.OutputsF64(ctypes.c_void_p(16), None, None, None)
+    f = lib.noa_dcs_recoil_integrals_f64
+    assert lib.noa_dcs_recoil_workspace_bytes(100, 180) == 100 * 180 * 48
+    assert lib.noa_dcs_recoil_workspace_bytes(10 ** 6, 180) == 16384 * 180 * 48
+    assert f(ctypes.c_void_p(16), 10, 0.05, ctypes.byref(rock), 0.1, 180, ctypes.byref(o), None, None, 0,
+             None) == _lib.NOA_DCS_EINVAL and "workspace" in _lib.last_error()
+    assert f(ctypes.c_void_p(16), 10, 1.0, ctypes.byref(rock), 0.1, 180, ctypes.byref(o), None, None, 0,
+             None) == _lib.NOA_DCS_EINVAL and "xlow" in _lib.last_error()
+    assert f(ctypes.c_void_p(16), 10, 0.05, ctypes.byref(rock), 0.1, 0, ctypes.byref(o), None, None, 0,
+             None) == _lib.NOA_DCS_EINVAL
+    assert f(ctypes.c_void_p(16), 10, 0.05, ctypes.byref(rock), 0.1, 180, None, None, None, 0, None) == _lib.NOA_DCS_OK

@@ -198,3 +198,38 @@ def test_opcount_counts_per_pair_work():
     # and two for zeta (gamma = 1 + K/m > 35 here)
     c = oracle.opcount(1, np.array([10.0]), np.array([1.0]), STANDARD_ROCK, MUON_MASS)
     assert c["exp"] == 8 and c["log"] == 1 + 7 * 8 + 2
+
+
+@pytest.mark.parametrize("name", ["rock", "hydrogen"])
+@pytest.mark.parametrize("min_points", [180, 7, 61])
+def test_recoil_integrals_bitwise_equal_reference_golden(golden, name, min_points):
+    el = _element(golden, name)
+    K = golden["recoil_K"]
+    exp = golden[f"recoil_{name}_mp{min_points}"]
+    for cel in (0, 1):
+        for p in range(4):
+            got = oracle.recoil(p, cel, K, 0.05, el, MUON_MASS, min_points)
+            assert np.array_equal(got.view(np.uint64), exp[cel][p].view(np.uint64)), (cel, p)
+
+
+def test_recoil_integral_closed_form_ionisation():
+    # below the switch the ionisation DEL is the analytic close-collision
+    # integral (dcs.hh:444-497); check it against a fine numerical quadrature
+    # of the oracle's own ionisation DCS over q in [max(0.62 I, K xlow), Wmax]
+    el, m, me = STANDARD_ROCK, MUON_MASS, ELECTRON_MASS
+    K = np.array([0.2, 0.5, 1.0])
+    got = oracle.recoil(3, 0, K, 0.05, el, m)
+    for k, g in zip(K, got):
+        P2 = k * (k + 2 * m)
+        E = k + m
+        wmax = 2 * me * P2 / (m * m + me * (me + 2 * E))
+        if wmax < 0.05 * k:  # the exit of dcs.hh:482-483
+            assert g == 0.0
+            continue
+        lo = max(0.62 * el.I, k * 0.05)
+        t = np.linspace(np.log(lo), np.log(wmax), 20001)
+        q = np.exp(t)
+        # close-interaction DCS without the radiative correction (K is below its switch)
+        cs = 1.535336e-5 * E * el.Z / el.A * (0.5 / P2 + (1 / q) * (-1 / wmax + (E * E / P2) / q))
+        num = np.trapz(cs * q, t) / E  # over t = ln q: the DEL is (1/E) int dcs dq
+        assert g == pytest.approx(num, rel=1e-6)
