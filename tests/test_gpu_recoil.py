@@ -5,7 +5,7 @@ import pytest
 import torch
 
 import oracle
-from noa_amd import dcs
+from noa_amd import _lib, dcs
 from noa_amd.physics import ELECTRON_MASS, MUON_MASS, STANDARD_ROCK, TAU_MASS, AtomicElement
 from tests.parity import compare
 
@@ -46,5 +46,5 @@ def test_reference_call_shape_and_chunking(device):
     exp2 = oracle.recoil(3, 1, Kh, 0.05, STANDARD_ROCK, MUON_MASS, 180, threads=0)
     r = compare(res2.cpu().numpy(), exp2)
     assert r["zero_mismatch"] == 0 and r["max_rel"] <= 1e-10, r
-    with pytest.raises(ValueError):
+    with pytest.raises(_lib.NoaDcsError, match="xlow"):
         dcs.recoil_integrals(K, 1.5, STANDARD_ROCK, MUON_MASS, dels=(res, None, None, None))

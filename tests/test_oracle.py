@@ -231,5 +231,5 @@ def test_recoil_integral_closed_form_ionisation():
         q = np.exp(t)
         # close-interaction DCS without the radiative correction (K is below its switch)
         cs = 1.535336e-5 * E * el.Z / el.A * (0.5 / P2 + (1 / q) * (-1 / wmax + (E * E / P2) / q))
-        num = np.trapz(cs * q, t) / E  # over t = ln q: the DEL is (1/E) int dcs dq
+        num = np.trapezoid(cs * q, t) / E  # over t = ln q: the DEL is (1/E) int dcs dq
         assert g == pytest.approx(num, rel=1e-6)
