@@ -85,9 +85,21 @@ def read_traffic(kernel_prefix: str):
     return vals.get("hbm")
 
 
-def cpu_baseline(n_total: int) -> dict:
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(n_total: int, min_seconds: float = 10.0) -> dict:
     """The reference's OpenMP path (oracle/_ref, compiled from the reference
-    headers) or, where it is absent, the restated oracle, on this host."""
+    headers) or, where it is absent, the restated oracle, on this host: the
+    whole workload, repeated until at least `min_seconds` of CPU work, plus a
+    one-thread figure on its first 2^20 pairs."""
     import numpy as np
 
     import oracle
@@ -95,24 +107,34 @@ def cpu_baseline(n_total: int) -> dict:
     from noa_amd.physics import MUON_MASS, STANDARD_ROCK
     ref = oracle.load_ref()
     kind = "reference" if ref is not None else "port"
-    n = min(n_total, 1 << 22)
+    n = n_total
     K, q = workload.pairs(n, seed=0)
 
-    def run(m):
+    def run(m, parallel=True):
         if kind == "reference":
             for p in range(4):
-                oracle.ref_map(p, K[:m], q[:m], STANDARD_ROCK, MUON_MASS, parallel=True)
+                oracle.ref_map(p, K[:m], q[:m], STANDARD_ROCK, MUON_MASS, parallel=parallel)
         else:
-            oracle.dcs_map_all(K[:m], q[:m], STANDARD_ROCK, MUON_MASS, threads=CPU_THREADS)
+            oracle.dcs_map_all(K[:m], q[:m], STANDARD_ROCK, MUON_MASS, threads=CPU_THREADS if parallel else 1)
 
     run(1 << 14)  # warm-up (thread team, page faults)
-    t = time.perf_counter()
-    run(n)
-    dt = time.perf_counter() - t
-    return {"value": n / dt, "unit": "evals/s", "cores": int(os.environ.get("OMP_NUM_THREADS", CPU_THREADS)),
+    reps, t = 0, time.perf_counter()
+    while True:
+        run(n)
+        reps += 1
+        dt = time.perf_counter() - t
+        if dt >= min_seconds or reps >= 64:
+            break
+    n1 = min(n, 1 << 20)
+    t1 = time.perf_counter()
+    run(n1, parallel=False)
+    dt1 = time.perf_counter() - t1
+    return {"value": n * reps / dt, "unit": "evals/s", "cores": int(os.environ.get("OMP_NUM_THREADS", CPU_THREADS)),
             "kind": kind,
-            "sample": f"first {n} pairs of the same seeded workload (2^{int(np.log2(n))}), all 4 processes, "
-                      f"one OpenMP map per process as dcs::pvmap does, {dt:.2f} s"}
+            "sample": f"the whole {n}-pair workload (2^{int(np.log2(n))}) x {reps} repetitions, all 4 processes, "
+                      f"one OpenMP map per process as dcs::pvmap does, {dt:.2f} s",
+            "one_thread": {"value": n1 / dt1, "sample": f"first {n1} pairs, serial dcs::vmap, {dt1:.2f} s"},
+            "cpu_model": cpu_model(), "host_cpus": os.cpu_count()}
 
 
 def main() -> None:
@@ -185,6 +207,25 @@ def main() -> None:
         forms["f32mixed"] = {"evals_per_s": n * args.steps / w32, "kernel_ms": k32,
                              "speedup_vs_f64": wall / w32, "max_rel_err_vs_f64": max_rel_vs(o32, ref)}
         del o32
+        # per-process rates: one output requested, the other processes skipped
+        per = {}
+        for pi, name in enumerate(("bremsstrahlung", "pair_production", "photonuclear", "ionisation")):
+            sel = [None] * 4
+            sel[pi] = outs[pi]
+            wp, kp = timed_launches(lambda: dcs.eval_f64(K, q, STANDARD_ROCK, MUON_MASS, *sel),
+                                    max(5, args.steps // 5), 2, stream)
+            per[name] = {"evals_per_s": n * max(5, args.steps // 5) / wp, "kernel_ms": kp}
+        forms["per_process_f64"] = per
+        # end to end through the LibTorch layer (include/noa/pms/dcs_hip.hh):
+        # dcs::hip::map_all allocates its four outputs on every call
+        try:
+            from noa_amd import _muon_dcs
+            el = STANDARD_ROCK
+            f = lambda: _muon_dcs.all_processes(K, q, el.A, el.I, el.Z, MUON_MASS)  # noqa: E731
+            wl, kl = timed_launches(f, args.steps, args.warmup, stream)
+            forms["libtorch_map_all_f64"] = {"evals_per_s": n * args.steps / wl, "ms_per_call": kl}
+        except ImportError as e:
+            forms["libtorch_map_all_f64"] = {"skipped": str(e)}
         try:
             from noa_amd import surrogate
             forms["surrogate"] = surrogate.bench_entry(K, q, ref, args.steps, args.warmup, stream, wall)

@@ -6,6 +6,10 @@
 //   dcs::map(f)(K, q, element, mass)            src/noa/pms/dcs.hh:50-60
 //   dcs::pvmap / dcs::pmap                      src/noa/pms/dcs.hh:62-87
 //   dcs::cuda::vmap_bremsstrahlung / map_...    src/noa/pms/dcs.hh:1004-1019
+//   dcs::vmap_integral(dcs::recoil_integral(f, dcs::del_integrand / cel_integrand))
+//                                               src/noa/pms/dcs.hh:89-130, 955-1001
+//   dcs::coulomb_data / coulomb_transport / hard_scattering / soft_scattering
+//                                               src/noa/pms/dcs.hh:604-950
 // for f in {bremsstrahlung, pair_production, photonuclear, ionisation}
 // (src/noa/pms/dcs.hh:134-443). The types and constants of
 // src/noa/pms/physics.hh:29-76 that these calls take are declared here too.
@@ -22,6 +26,7 @@
 #include <c10/core/DeviceGuard.h>
 #include <c10/hip/HIPStream.h>
 
+#include <algorithm>
 #include <array>
 #include <cstdint>
 
@@ -162,6 +167,137 @@ inline Scalar Process::operator()(const Scalar& K, const Scalar& q, const Atomic
     const auto k = torch::full({1}, K, opts);
     const auto qq = torch::full({1}, q, opts);
     return map(*this)(k, qq, el, mass).item<Scalar>();
+}
+
+// ---- recoil integrals (dcs.hh:89-130, 444-497, 955-1001) ----
+
+struct Integrand {
+    bool cel;
+};
+inline constexpr Integrand del_integrand{false};  // dcs.hh:107-109: dcs * q
+inline constexpr Integrand cel_integrand{true};   // dcs.hh:111-113: dcs * q^2
+
+struct RecoilIntegral {
+    Process process;
+    Integrand integrand;
+};
+
+// dcs.hh:89-105 (with the ionisation specialisations of dcs.hh:955-1001).
+inline RecoilIntegral recoil_integral(const Process& dcs_func, const Integrand& integrand) {
+    return RecoilIntegral{dcs_func, integrand};
+}
+
+// dcs.hh:115-130: f(result, K, xlow, element, mass, min_points) in place.
+inline auto vmap_integral(const RecoilIntegral& cs) {
+    return [cs](const Calculation& result, const Energies& kinetic_energies, const Scalar& xlow,
+                const AtomicElement& element, const ParticleMass& mass, const Index min_points) {
+        detail::check_input(kinetic_energies, "kinetic_energies");
+        detail::check_output(result, kinetic_energies, torch::kFloat64);
+        const c10::DeviceGuard guard(kinetic_energies.device());
+        const int64_t n = kinetic_energies.numel();
+        const int64_t bytes = noa_dcs_recoil_workspace_bytes(n, min_points);
+        const auto ws = torch::empty({std::max<int64_t>(bytes, 8)},
+                                     torch::TensorOptions().dtype(torch::kUInt8).device(kinetic_energies.device()));
+        double* p[4] = {nullptr, nullptr, nullptr, nullptr};
+        p[cs.process.id] = result.data_ptr<double>();
+        const noa_dcs_outputs_f64 o{p[0], p[1], p[2], p[3]};
+        const noa_dcs_element e{element.A, element.I, element.Z};
+        detail::raise(noa_dcs_recoil_integrals_f64(kinetic_energies.data_ptr<double>(), n, xlow, &e, mass, min_points,
+                                                   cs.integrand.cel ? nullptr : &o, cs.integrand.cel ? &o : nullptr,
+                                                   ws.data_ptr(), bytes, detail::stream_of(kinetic_energies)));
+    };
+}
+
+// ---- Coulomb scattering (dcs.hh:499-950) ----
+
+using CMLorentz = torch::Tensor;         // [n, 2] (physics.hh:101)
+using AngularCutoff = torch::Tensor;     // [1] or [n]
+using HSMeanFreePath = torch::Tensor;    // [n]
+using TransportCoefs = torch::Tensor;    // [n, 2]
+using InvLambdas = torch::Tensor;        // [n] or [nel, n]
+using ScreeningFactors = torch::Tensor;  // [n, 9]
+using FSpins = torch::Tensor;            // [n] or [nel, n]
+
+namespace detail {
+
+inline void check_f64(const torch::Tensor& t, const char* name, int64_t numel, const torch::Device& dev) {
+    check_input(t, name);
+    TORCH_CHECK(t.device() == dev, name, " is on ", t.device(), ", expected ", dev);
+    TORCH_CHECK(t.numel() == numel, name, " must hold ", numel, " values, got ", t.numel());
+}
+
+}  // namespace detail
+
+// dcs.hh:604-622.
+inline void coulomb_data(const CMLorentz& fCM, const ScreeningFactors& screening, const FSpins& fspin,
+                         const InvLambdas& invlambda, const Energies& kinetic_energies, const AtomicElement& element,
+                         const ParticleMass& mass) {
+    detail::check_input(kinetic_energies, "kinetic_energies");
+    const int64_t n = kinetic_energies.numel();
+    const auto dev = kinetic_energies.device();
+    detail::check_f64(fCM, "fCM", 2 * n, dev);
+    detail::check_f64(screening, "screening", 9 * n, dev);
+    detail::check_f64(fspin, "fspin", n, dev);
+    detail::check_f64(invlambda, "invlambda", n, dev);
+    const c10::DeviceGuard guard(dev);
+    const noa_dcs_element e{element.A, element.I, element.Z};
+    detail::raise(noa_dcs_coulomb_data_f64(kinetic_energies.data_ptr<double>(), n, &e, mass, fCM.data_ptr<double>(),
+                                           screening.data_ptr<double>(), fspin.data_ptr<double>(),
+                                           invlambda.data_ptr<double>(), detail::stream_of(kinetic_energies)));
+}
+
+// dcs.hh:676-694.
+inline void coulomb_transport(const TransportCoefs& coefficients, const ScreeningFactors& screening,
+                              const FSpins& fspin, const AngularCutoff& mu) {
+    const int64_t n = fspin.numel();
+    const auto dev = fspin.device();
+    detail::check_f64(fspin, "fspin", n, dev);
+    detail::check_f64(screening, "screening", 9 * n, dev);
+    detail::check_f64(coefficients, "coefficients", 2 * n, dev);
+    TORCH_CHECK(mu.numel() == 1 || mu.numel() == n, "mu must hold 1 or ", n, " values");
+    detail::check_f64(mu, "mu", mu.numel(), dev);
+    const c10::DeviceGuard guard(dev);
+    detail::raise(noa_dcs_coulomb_transport_f64(screening.data_ptr<double>(), fspin.data_ptr<double>(),
+                                                mu.data_ptr<double>(), mu.numel(), n,
+                                                coefficients.data_ptr<double>(), detail::stream_of(fspin)));
+}
+
+// dcs.hh:842-872: invlambdas is [nel, nkin]; the other per-element arrays
+// [nel, nkin, ...].
+inline void hard_scattering(const AngularCutoff& mu0, const HSMeanFreePath& lb_h, const TransportCoefs& coefficients,
+                            const CMLorentz& transform, const ScreeningFactors& screening,
+                            const InvLambdas& invlambdas, const FSpins& fspins) {
+    TORCH_CHECK(invlambdas.dim() == 2, "invlambdas must be [nel, nkin]");
+    const int64_t nel = invlambdas.size(0), nkin = invlambdas.size(1);
+    const auto dev = invlambdas.device();
+    detail::check_f64(mu0, "mu0", nkin, dev);
+    detail::check_f64(lb_h, "lb_h", nkin, dev);
+    detail::check_f64(coefficients, "coefficients", nel * nkin * 2, dev);
+    detail::check_f64(transform, "transform", nel * nkin * 2, dev);
+    detail::check_f64(screening, "screening", nel * nkin * 9, dev);
+    detail::check_f64(invlambdas, "invlambdas", nel * nkin, dev);
+    detail::check_f64(fspins, "fspins", nel * nkin, dev);
+    const c10::DeviceGuard guard(dev);
+    detail::raise(noa_dcs_hard_scattering_f64(nel, nkin, coefficients.data_ptr<double>(), transform.data_ptr<double>(),
+                                              screening.data_ptr<double>(), invlambdas.data_ptr<double>(),
+                                              fspins.data_ptr<double>(), mu0.data_ptr<double>(),
+                                              lb_h.data_ptr<double>(), detail::stream_of(invlambdas)));
+}
+
+// dcs.hh:940-950.
+inline void soft_scattering(const Calculation& ms1, const Energies& kinetic_energies, const AtomicElement& element,
+                            const ParticleMass& mass) {
+    detail::check_input(kinetic_energies, "kinetic_energies");
+    detail::check_output(ms1, kinetic_energies, torch::kFloat64);
+    const c10::DeviceGuard guard(kinetic_energies.device());
+    const int64_t n = kinetic_energies.numel();
+    const int64_t bytes = noa_dcs_soft_scattering_workspace_bytes(n);
+    const auto ws = torch::empty({std::max<int64_t>(bytes, 8)},
+                                 torch::TensorOptions().dtype(torch::kUInt8).device(kinetic_energies.device()));
+    const noa_dcs_element e{element.A, element.I, element.Z};
+    detail::raise(noa_dcs_soft_scattering_f64(kinetic_energies.data_ptr<double>(), n, &e, mass,
+                                              ms1.data_ptr<double>(), ws.data_ptr(), bytes,
+                                              detail::stream_of(kinetic_energies)));
 }
 
 namespace hip {

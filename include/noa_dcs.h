@@ -103,6 +103,34 @@ int noa_dcs_recoil_integrals_f64(const double* K, int64_t n, double xlow, const 
                                  const noa_dcs_outputs_f64* cel, void* workspace, int64_t workspace_bytes,
                                  hipStream_t stream);
 
+/* ---- Coulomb scattering (src/noa/pms/dcs.hh:499-950) ----
+ * Arrays are row-major like the reference's tensors: fCM [n][2],
+ * screening [n][9], G [n][2]; for a mixture of nel elements the per-element
+ * arrays are [nel][nkin][...] (element-major).
+ *
+ * Replaces dcs::coulomb_data (dcs.hh:604-622): per kinetic energy, the
+ * centre-of-mass factors, the spin factor, the 3 screening parameters and 6
+ * pole-reduction coefficients, and the inverse Wentzel path.
+ */
+int noa_dcs_coulomb_data_f64(const double* K, int64_t n, const noa_dcs_element* element, double mass, double* fCM,
+                             double* screening, double* fspin, double* invlambda, hipStream_t stream);
+/* Replaces dcs::coulomb_transport (dcs.hh:676-694): first and second transport
+ * coefficients G at the cutoff mu (n_mu = 1: one mu for all; n_mu = n: per K). */
+int noa_dcs_coulomb_transport_f64(const double* screening, const double* fspin, const double* mu, int64_t n_mu,
+                                  int64_t n, double* G, hipStream_t stream);
+/* Replaces dcs::hard_scattering (dcs.hh:842-872): per kinetic energy, the
+ * hard-scattering cutoff angle mu0 (Ridders' root, numerics.hh:154-214) and
+ * mean free path lb_h of a mixture of nel elements. */
+int noa_dcs_hard_scattering_f64(int64_t nel, int64_t nkin, const double* G, const double* fCM,
+                                const double* screening, const double* invlambda, const double* fspin, double* mu0,
+                                double* lb_h, hipStream_t stream);
+/* Replaces dcs::soft_scattering (dcs.hh:940-950): the first transverse
+ * transport coefficient of soft ionisation and photonuclear interactions.
+ * workspace is a device buffer of noa_dcs_soft_scattering_workspace_bytes(n). */
+int64_t noa_dcs_soft_scattering_workspace_bytes(int64_t n);
+int noa_dcs_soft_scattering_f64(const double* K, int64_t n, const noa_dcs_element* element, double mass,
+                                double* result, void* workspace, int64_t workspace_bytes, hipStream_t stream);
+
 /* ---- batched uneven elements: one persistent launch, captured in a hipGraph ----
  * Element e owns pairs [offsets[e], offsets[e+1]) of the concatenated K, q
  * and output arrays. The reference has no batched form: a caller loops
@@ -168,8 +196,10 @@ int noa_dcs_mlp_train(noa_dcs_mlp* mlp, const float* X, const float* T, const fl
 int noa_dcs_mlp_infer_fused(noa_dcs_mlp* mlp, const float* X, int64_t n, float* Y, hipStream_t stream);
 
 typedef struct noa_dcs_surrogate_cfg {
-    float out_mean[4];      /* de-standardisation of the outputs: */
-    float out_std[4];       /*   ln(A * DCS_p) = out_mean[p] + out_std[p] * y_p */
+    float out_mean[4];      /* de-standardisation of the outputs: ln(A * DCS_p) = */
+    float out_std[4];       /*   out_mean[p] + sum_k trend[p][k] phi_k(x) + out_std[p] * y_p with the */
+    float trend[4][13];     /*   cubic basis phi(x) = (a, b, a^2, b^2, ab, a^3, b^3, a^2 b, a b^2, z, za, zb,
+                                 z^2) of the features x = (a, b, z) */
     const uint8_t* flag_map; /* device [n_elements][bins][bins] over (log10 K in [-3,6], log10 nu in [-6,0]);
                                 non-zero = evaluate that cell exactly; may be NULL */
     int32_t bins;

@@ -52,7 +52,8 @@ class BatchConfig(ctypes.Structure):
 
 
 class SurrogateConfig(ctypes.Structure):
-    _fields_ = [("out_mean", ctypes.c_float * 4), ("out_std", ctypes.c_float * 4), ("flag_map", ctypes.c_void_p),
+    _fields_ = [("out_mean", ctypes.c_float * 4), ("out_std", ctypes.c_float * 4),
+                ("trend", (ctypes.c_float * 13) * 4), ("flag_map", ctypes.c_void_p),
                 ("bins", ctypes.c_int32), ("recheck_per_mille", ctypes.c_int32), ("seed", ctypes.c_uint64)]
 
 
@@ -76,6 +77,11 @@ SIGNATURES = {
     "noa_dcs_batch_info": (ctypes.c_int, [_P, ctypes.POINTER(_I32), ctypes.POINTER(_I32), ctypes.POINTER(_I64)]),
     "noa_dcs_batch_destroy": (ctypes.c_int, [_P]),
     "noa_dcs_fill_workload": (ctypes.c_int, [_P, _P, _I64, _U64, _U64, _D, _D, _D, _D, _P]),
+    "noa_dcs_coulomb_data_f64": (ctypes.c_int, [_P, _I64, ctypes.POINTER(Element), _D, _P, _P, _P, _P, _P]),
+    "noa_dcs_coulomb_transport_f64": (ctypes.c_int, [_P, _P, _P, _I64, _I64, _P, _P]),
+    "noa_dcs_hard_scattering_f64": (ctypes.c_int, [_I64, _I64, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "noa_dcs_soft_scattering_workspace_bytes": (ctypes.c_int64, [_I64]),
+    "noa_dcs_soft_scattering_f64": (ctypes.c_int, [_P, _I64, ctypes.POINTER(Element), _D, _P, _P, _I64, _P]),
     "noa_dcs_recoil_workspace_bytes": (ctypes.c_int64, [_I64, _I32]),
     "noa_dcs_recoil_integrals_f64": (ctypes.c_int, [_P, _I64, _D, ctypes.POINTER(Element), _D, _I32,
                                                     ctypes.POINTER(OutputsF64), ctypes.POINTER(OutputsF64), _P,

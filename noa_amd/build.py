@@ -19,7 +19,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libnoa_dcs.so")
 BUILD_DIR = os.path.join(HERE, "_build")
 
-HIP_SOURCES = ["dcs_eval.hip", "dcs_batch.hip", "dcs_recoil.hip", "surrogate_train.hip", "surrogate_infer.hip"]
+HIP_SOURCES = ["dcs_eval.hip", "dcs_batch.hip", "dcs_recoil.hip", "dcs_coulomb.hip", "surrogate_train.hip", "surrogate_infer.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [

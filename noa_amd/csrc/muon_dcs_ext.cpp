@@ -46,6 +46,40 @@ double scalar(int process, double K, double q, double A, double I, int Z, double
     return procs[process](K, q, AtomicElement{A, I, Z}, mass);
 }
 
+// dcs::vmap_integral(dcs::recoil_integral(process, del/cel_integrand)).
+torch::Tensor recoil_integral(int process, bool cel, torch::Tensor kinetic_energies, double xlow, double A, double I,
+                              int Z, double mass, int min_points) {
+    TORCH_CHECK(process >= 0 && process < 4, "process must be 0..3");
+    const dcs::Process procs[4] = {dcs::bremsstrahlung, dcs::pair_production, dcs::photonuclear, dcs::ionisation};
+    auto result = torch::zeros_like(kinetic_energies);
+    dcs::vmap_integral(dcs::recoil_integral(procs[process], cel ? dcs::cel_integrand : dcs::del_integrand))(
+        result, kinetic_energies, xlow, AtomicElement{A, I, Z}, mass, min_points);
+    return result;
+}
+
+// dcs::coulomb_data followed by dcs::coulomb_transport at mu = 1 and
+// dcs::hard_scattering (the chain of test/unit/test-dcs-calc.cc:134-166):
+// returns fCM, screening, fspin, invlambda, G, mu0, lb_h.
+std::vector<torch::Tensor> coulomb_chain(torch::Tensor kinetic_energies, double A, double I, int Z, double mass) {
+    const int64_t n = kinetic_energies.numel();
+    const auto opt = kinetic_energies.options();
+    auto fCM = torch::zeros({n, 2}, opt), screen = torch::zeros({n, 9}, opt);
+    auto fspin = torch::zeros({n}, opt), invlambda = torch::zeros({n}, opt), G = torch::zeros({n, 2}, opt);
+    auto mu0 = torch::zeros({n}, opt), lb_h = torch::zeros({n}, opt);
+    dcs::coulomb_data(fCM, screen, fspin, invlambda, kinetic_energies, AtomicElement{A, I, Z}, mass);
+    dcs::coulomb_transport(G, screen, fspin, torch::ones({1}, opt));
+    dcs::hard_scattering(mu0, lb_h, G.view({1, n, 2}), fCM.view({1, n, 2}), screen.view({1, n, 9}),
+                         invlambda.view({1, n}), fspin.view({1, n}));
+    return {fCM, screen, fspin, invlambda, G, mu0, lb_h};
+}
+
+// dcs::soft_scattering.
+torch::Tensor soft_scattering(torch::Tensor kinetic_energies, double A, double I, int Z, double mass) {
+    auto result = torch::zeros_like(kinetic_energies);
+    dcs::soft_scattering(result, kinetic_energies, AtomicElement{A, I, Z}, mass);
+    return result;
+}
+
 void serialise(torch::Tensor tensor, std::string path) { torch::save(tensor, path); }
 
 }  // namespace
@@ -64,5 +98,11 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
     m.def("vmap_process", &vmap_process, py::call_guard<py::gil_scoped_release>(),
           "dcs::vmap(process)(result, K, q, element, mass)");
     m.def("scalar", &scalar, "One (K, q) pair through the scalar functor form");
+    m.def("recoil_integral", &recoil_integral, py::call_guard<py::gil_scoped_release>(),
+          "DEL (cel=False) or CEL (cel=True) recoil integral of one process");
+    m.def("coulomb_chain", &coulomb_chain, py::call_guard<py::gil_scoped_release>(),
+          "coulomb_data, coulomb_transport at mu = 1 and hard_scattering");
+    m.def("soft_scattering", &soft_scattering, py::call_guard<py::gil_scoped_release>(),
+          "First transverse transport coefficient of the soft processes");
     m.def("serialise", &serialise, py::call_guard<py::gil_scoped_release>(), "Save tensor to disk");
 }

@@ -192,6 +192,10 @@ __global__ __launch_bounds__(IWG) void mlp_fused_kernel(const float* __restrict_
 #pragma unroll
             for (int j = 0; j < IN; j++) x[nt][j] = g < n ? X[g * IN + j] : 0.f;
         }
+        // lane l's own query qb + l is N-tile l/32's column l%32
+        float xs[IN];
+#pragma unroll
+        for (int j = 0; j < IN; j++) xs[j] = lane < 32 ? x[0][j] : x[1][j];
         float out[2][OUT];
         mlp_wave(L, x, out);
         if (lane < 32)
@@ -217,9 +221,12 @@ struct SurItem {
     int32_t element;
 };
 
+constexpr int NTREND = 13;
+
 struct SurCfg {
     float mean[OUT];
     float stdv[OUT];
+    float trend[OUT][NTREND];
     const uint8_t* flag_map;
     int32_t bins;
     uint32_t recheck_per_mille;
@@ -276,6 +283,10 @@ __global__ __launch_bounds__(IWG) void surrogate_dcs_kernel(
                 for (int j = 0; j < IN; j++) x[nt][j] = 0.f;
             }
         }
+        // lane l's own query qb + l is N-tile l/32's column l%32
+        float xs[IN];
+#pragma unroll
+        for (int j = 0; j < IN; j++) xs[j] = lane < 32 ? x[0][j] : x[1][j];
         float out[2][OUT];
         mlp_wave(L, x, out);
         // lane l takes query qb + l: N-tile 1's outputs move from lane l-32
@@ -303,10 +314,17 @@ __global__ __launch_bounds__(IWG) void surrogate_dcs_kernel(
             recheck_only = sample && !ood && !cell;
             const bool act[OUT] = {true, F64::pair_active(k, qq, E.c), F64::photo_active(k, qq, E.c),
                                    ion_active(k, qq, E.c)};
+            const float a = xs[0], b = xs[1], z = xs[2];
+            const float phi[NTREND] = {a,         b,         a * a, b * b, a * b, a * a * a, b * b * b,
+                                       a * a * b, a * b * b, z,     z * a, z * b, z * z};
             float* dst[OUT] = {o.b, o.p, o.ph, o.i};
 #pragma unroll
-            for (int p = 0; p < OUT; p++)
-                if (dst[p]) dst[p][g] = act[p] ? __expf(fmaf(cfg.stdv[p], t[p], cfg.mean[p]) - E.ln_a) : 0.f;
+            for (int p = 0; p < OUT; p++) {
+                float lnad = fmaf(cfg.stdv[p], t[p], cfg.mean[p]);
+#pragma unroll
+                for (int j = 0; j < NTREND; j++) lnad = fmaf(cfg.trend[p][j], phi[j], lnad);
+                if (dst[p]) dst[p][g] = act[p] ? __expf(lnad - E.ln_a) : 0.f;
+            }
         }
         // append flagged queries: one global atomic per wave
         const unsigned long long m = __ballot(flag);
@@ -454,6 +472,7 @@ int noa_dcs_surrogate_eval(noa_dcs_mlp* net, const noa_dcs_element* elements, co
     for (int p = 0; p < OUT; p++) {
         c.mean[p] = cfg->out_mean[p];
         c.stdv[p] = cfg->out_std[p];
+        for (int j = 0; j < NTREND; j++) c.trend[p][j] = cfg->trend[p][j];
     }
     c.flag_map = cfg->flag_map;
     c.bins = cfg->bins;

@@ -21,6 +21,7 @@ from __future__ import annotations
 import ctypes
 from types import SimpleNamespace
 
+import numpy as np
 import torch
 
 from . import _lib
@@ -28,7 +29,9 @@ from .physics import AtomicElement, MUON_MASS, STANDARD_ROCK  # noqa: F401  (re-
 
 __all__ = [
     "Process", "bremsstrahlung", "pair_production", "photonuclear", "ionisation", "PROCESSES",
-    "vmap", "map", "pvmap", "pmap", "hip", "relative_error", "mean_error",
+    "vmap", "map", "pvmap", "pmap", "hip", "relative_error", "mean_error", "del_integrand", "cel_integrand",
+    "recoil_integral", "vmap_integral", "recoil_integrals", "coulomb_data", "coulomb_transport",
+    "hard_scattering", "soft_scattering",
 ]
 
 
@@ -246,6 +249,93 @@ def vmap_integral(cs_integral):
         else:
             recoil_integrals(kinetic_energies, xlow, element, mass, min_points, dels=outs)
     return run
+
+
+# ---- Coulomb scattering (src/noa/pms/dcs.hh:499-950) ----
+
+def _check_f64(name: str, t, shape, device) -> None:
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{name} must be a torch.Tensor")
+    if t.dtype != torch.float64:
+        raise ValueError(f"{name} must be float64 (got {t.dtype})")
+    if t.device != device or t.device.type != "cuda":
+        raise ValueError(f"{name} must be on {device} (got {t.device}); there is no CPU path")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    if shape is not None and t.numel() != int(np.prod(shape)):
+        raise ValueError(f"{name} must hold {int(np.prod(shape))} values as {tuple(shape)} (got {tuple(t.shape)})")
+
+
+def coulomb_data(fCM: torch.Tensor, screening: torch.Tensor, fspin: torch.Tensor, invlambda: torch.Tensor,
+                 kinetic_energies: torch.Tensor, element: AtomicElement, mass: float) -> None:
+    """dcs.hh:604-622: fills fCM [n, 2], screening [n, 9], fspin [n] and
+    invlambda [n] for the n kinetic energies."""
+    K = kinetic_energies
+    _check_inputs(K, K)
+    n = K.numel()
+    for name, t, shape in (("fCM", fCM, (n, 2)), ("screening", screening, (n, 9)), ("fspin", fspin, (n,)),
+                           ("invlambda", invlambda, (n,))):
+        _check_f64(name, t, shape, K.device)
+    el = _element(element)
+    with torch.cuda.device(K.device):
+        _lib.check(_lib.lib().noa_dcs_coulomb_data_f64(_ptr(K), n, ctypes.byref(el), float(mass), _ptr(fCM),
+                                                       _ptr(screening), _ptr(fspin), _ptr(invlambda), _stream(K)))
+
+
+def coulomb_transport(coefficients: torch.Tensor, screening: torch.Tensor, fspin: torch.Tensor,
+                      mu: torch.Tensor) -> None:
+    """dcs.hh:676-694: transport coefficients [n, 2] at the cutoff mu (one
+    value for all, or one per kinetic energy)."""
+    n = fspin.numel()
+    dev = fspin.device
+    _check_f64("fspin", fspin, (n,), dev)
+    _check_f64("screening", screening, (n, 9), dev)
+    _check_f64("coefficients", coefficients, (n, 2), dev)
+    _check_f64("mu", mu, None, dev)
+    if mu.numel() not in (1, n):
+        raise ValueError(f"mu must hold 1 or {n} values (got {mu.numel()})")
+    with torch.cuda.device(dev):
+        _lib.check(_lib.lib().noa_dcs_coulomb_transport_f64(_ptr(screening), _ptr(fspin), _ptr(mu), mu.numel(), n,
+                                                            _ptr(coefficients), _stream(fspin)))
+
+
+def hard_scattering(mu0: torch.Tensor, lb_h: torch.Tensor, coefficients: torch.Tensor, transform: torch.Tensor,
+                    screening: torch.Tensor, invlambdas: torch.Tensor, fspins: torch.Tensor) -> None:
+    """dcs.hh:842-872: cutoff angle mu0 [nkin] and mean free path lb_h [nkin]
+    of elastic hard scatters in a mixture; invlambdas is [nel, nkin] and the
+    other per-element arrays [nel, nkin, ...] (a 1-D invlambdas is one
+    element)."""
+    if invlambdas.dim() == 1:
+        nel, nkin = 1, invlambdas.numel()
+    elif invlambdas.dim() == 2:
+        nel, nkin = invlambdas.shape
+    else:
+        raise ValueError("invlambdas must be [nel, nkin] (or [nkin])")
+    dev = invlambdas.device
+    for name, t, shape in (("mu0", mu0, (nkin,)), ("lb_h", lb_h, (nkin,)), ("coefficients", coefficients,
+                                                                            (nel, nkin, 2)),
+                           ("transform", transform, (nel, nkin, 2)), ("screening", screening, (nel, nkin, 9)),
+                           ("invlambdas", invlambdas, (nel, nkin)), ("fspins", fspins, (nel, nkin))):
+        _check_f64(name, t, shape, dev)
+    with torch.cuda.device(dev):
+        _lib.check(_lib.lib().noa_dcs_hard_scattering_f64(nel, nkin, _ptr(coefficients), _ptr(transform),
+                                                          _ptr(screening), _ptr(invlambdas), _ptr(fspins),
+                                                          _ptr(mu0), _ptr(lb_h), _stream(invlambdas)))
+
+
+def soft_scattering(ms1: torch.Tensor, kinetic_energies: torch.Tensor, element: AtomicElement,
+                    mass: float) -> None:
+    """dcs.hh:940-950: first transverse transport coefficient of the soft
+    ionisation and photonuclear interactions at each kinetic energy."""
+    K = kinetic_energies
+    _check_inputs(K, K)
+    _check_output(ms1, K, torch.float64)
+    nbytes = _lib.lib().noa_dcs_soft_scattering_workspace_bytes(K.numel())
+    ws = torch.empty(max(nbytes, 8), dtype=torch.uint8, device=K.device)
+    el = _element(element)
+    with torch.cuda.device(K.device):
+        _lib.check(_lib.lib().noa_dcs_soft_scattering_f64(_ptr(K), K.numel(), ctypes.byref(el), float(mass),
+                                                          _ptr(ms1), _ptr(ws), nbytes, _stream(K)))
 
 
 def relative_error(computed: torch.Tensor, expected: torch.Tensor) -> torch.Tensor:

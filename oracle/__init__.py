@@ -151,3 +151,88 @@ def opcount(process: int, K, q, element, mass: float) -> dict:
 
 def max_threads() -> int:
     return load().oracle_max_threads()
+
+
+# ---- Coulomb scattering (coulomb_oracle.hh; reference dcs.hh:499-950) ----
+
+_I64 = ctypes.c_int64
+
+
+def _coulomb_fn(lib, prefix: str, name: str, argtypes):
+    f = getattr(lib, f"{prefix}_{name}")
+    f.argtypes = argtypes
+    f.restype = ctypes.c_int
+    return f
+
+
+def _c(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _coulomb_lib(use_ref: bool):
+    if use_ref:
+        ref = load_ref()
+        if ref is None:
+            raise FileNotFoundError(REF_PATH)
+        return ref, "ref"
+    return load(), "oracle"
+
+
+def coulomb_data(K, element, mass: float, use_ref: bool = False):
+    """(fCM [n,2], screening [n,9], fspin [n], invlambda [n]) of dcs::coulomb_data."""
+    lib, pre = _coulomb_lib(use_ref)
+    K = _c(K)
+    n = K.size
+    fCM, scr, fs, il = np.zeros((n, 2)), np.zeros((n, 9)), np.zeros(n), np.zeros(n)
+    f = _coulomb_fn(lib, pre, "coulomb_data", [_D, _I64, ctypes.c_double, ctypes.c_double, ctypes.c_int32,
+                                               ctypes.c_double, _D, _D, _D, _D])
+    assert f(_p(K), n, element.A, element.I, element.Z, mass, _p(fCM), _p(scr), _p(fs), _p(il)) == 0
+    return fCM, scr, fs, il
+
+
+def coulomb_transport(screening, fspin, mu, use_ref: bool = False):
+    """G [n,2] of dcs::coulomb_transport; mu has 1 or n values."""
+    lib, pre = _coulomb_lib(use_ref)
+    scr, fs, mu = _c(screening), _c(fspin), _c(np.atleast_1d(mu))
+    n = fs.size
+    G = np.zeros((n, 2))
+    f = _coulomb_fn(lib, pre, "coulomb_transport", [_D, _D, _D, _I64, _I64, _D])
+    assert f(_p(scr), _p(fs), _p(mu), mu.size, n, _p(G)) == 0
+    return G
+
+
+def hard_scattering(G, fCM, screening, invlambda, fspin, use_ref: bool = False):
+    """(mu0 [nkin], lb_h [nkin]) of dcs::hard_scattering over [nel, nkin, ...] arrays."""
+    lib, pre = _coulomb_lib(use_ref)
+    il = _c(invlambda)
+    if il.ndim == 1:
+        il = il[None, :]
+    nel, nkin = il.shape
+    G, fCM, scr, fs = _c(G), _c(fCM), _c(screening), _c(fspin)
+    mu0, lbh = np.zeros(nkin), np.zeros(nkin)
+    f = _coulomb_fn(lib, pre, "hard_scattering", [_I64, _I64, _D, _D, _D, _D, _D, _D, _D])
+    assert f(nel, nkin, _p(G), _p(fCM), _p(scr), _p(il), _p(fs), _p(mu0), _p(lbh)) == 0
+    return mu0, lbh
+
+
+def soft_scattering(K, element, mass: float, use_ref: bool = False, threads: int = 1) -> np.ndarray:
+    """dcs::soft_scattering at each K."""
+    K = _c(K)
+    out = np.zeros_like(K)
+    if use_ref:
+        lib, _ = _coulomb_lib(True)
+        f = _coulomb_fn(lib, "ref", "soft_scattering", [_D, _I64, ctypes.c_double, ctypes.c_double, ctypes.c_int32,
+                                                        ctypes.c_double, _D])
+        assert f(_p(K), K.size, element.A, element.I, element.Z, mass, _p(out)) == 0
+    else:
+        f = _coulomb_fn(load(), "oracle", "soft_scattering", [_D, _I64, ctypes.c_double, ctypes.c_double,
+                                                              ctypes.c_int32, ctypes.c_double, _D, ctypes.c_int])
+        assert f(_p(K), K.size, element.A, element.I, element.Z, mass, _p(out), threads) == 0
+    return out
+
+
+def photo_transport_ratio(K: float, q: float, mass: float) -> float:
+    f = load().oracle_photo_transport_ratio
+    f.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double]
+    f.restype = ctypes.c_double
+    return f(K, q, mass)

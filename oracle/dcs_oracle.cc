@@ -12,6 +12,7 @@
 
 #include <omp.h>
 
+#include "coulomb_oracle.hh"
 #include "dcs_oracle.hh"
 #include "opcount.hh"
 
@@ -117,6 +118,54 @@ int oracle_recoil_integral(int process, int cel, const double* K, int64_t n, dou
             result[i] = noa_oracle::process_recoil_integral<double>(process, cel != 0, K[i], xlow, el, mass, min_points);
     }
     return 0;
+}
+
+// ---- Coulomb scattering (coulomb_oracle.hh) ----
+
+// dcs.hh:604-622 over n kinetic energies: fCM [n][2], screen [n][9],
+// fspin [n], invlambda [n].
+int oracle_coulomb_data(const double* K, int64_t n, double A, double I, int32_t Z, double mass, double* fCM,
+                        double* screen, double* fspin, double* invlambda) {
+    const Element<double> el{A, I, Z};
+    for (int64_t i = 0; i < n; i++)
+        noa_oracle::coulomb_data(fCM + 2 * i, screen + noa_oracle::NSF * i, fspin + i, invlambda + i, K[i], el,
+                                 mass);
+    return 0;
+}
+
+// dcs.hh:676-694: G [n][2] for mu of n_mu = 1 (shared) or n values.
+int oracle_coulomb_transport(const double* screen, const double* fspin, const double* mu, int64_t n_mu, int64_t n,
+                             double* G) {
+    for (int64_t i = 0; i < n; i++)
+        noa_oracle::coulomb_transport_coefficients(G + 2 * i, screen + noa_oracle::NSF * i, fspin[i],
+                                                   mu[n_mu == 1 ? 0 : i]);
+    return 0;
+}
+
+// dcs.hh:842-872 over a mixture of nel elements and nkin kinetic energies.
+int oracle_hard_scattering(int64_t nel, int64_t nkin, const double* G, const double* fCM, const double* screen,
+                           const double* invlambda, const double* fspin, double* mu0, double* lb_h) {
+    for (int64_t i = 0; i < nkin; i++) {
+        const noa_oracle::MixtureView m{G, fCM, screen, invlambda, fspin, nel, nkin, i};
+        noa_oracle::coulomb_hard_scattering(mu0 + i, lb_h + i, m);
+    }
+    return 0;
+}
+
+// dcs.hh:940-950: first transverse transport coefficient of the soft
+// processes at each K.
+int oracle_soft_scattering(const double* K, int64_t n, double A, double I, int32_t Z, double mass, double* result,
+                           int threads) {
+    const Element<double> el{A, I, Z};
+    if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for default(none) shared(n, K, el, mass, result) schedule(dynamic, 16) if (threads != 1)
+    for (int64_t i = 0; i < n; i++) result[i] = noa_oracle::soft_scattering(K[i], el, mass);
+    return 0;
+}
+
+// The mu-integrated ratio at one photonuclear node (dcs.hh:910-932).
+double oracle_photo_transport_ratio(double K, double q, double mass) {
+    return noa_oracle::photonuclear_transport_ratio(K, q, mass);
 }
 
 }  // extern "C"

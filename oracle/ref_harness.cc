@@ -6,6 +6,7 @@
 // code: it only calls the reference's public adaptors through a C ABI so the
 // CPU oracle and the golden fixtures can be checked against the real thing.
 #include <cstdint>
+#include <vector>
 
 #include <noa/pms/dcs.hh>
 #include <noa/pms/physics.hh>
@@ -18,6 +19,10 @@ namespace {
 
 torch::Tensor wrap(const double* p, int64_t n) {
     return torch::from_blob(const_cast<double*>(p), {n}, torch::TensorOptions().dtype(torch::kFloat64));
+}
+
+torch::Tensor wrap_shape(const double* p, std::vector<int64_t> shape) {
+    return torch::from_blob(const_cast<double*>(p), shape, torch::TensorOptions().dtype(torch::kFloat64));
 }
 
 template <class F>
@@ -69,6 +74,41 @@ int ref_recoil_integral(int process, int cel, const double* K, int64_t n, double
         case 3: go(dcs::ionisation); break;
         default: return -1;
     }
+    return 0;
+}
+
+// ---- Coulomb scattering (dcs.hh:499-950) ----
+
+// dcs::coulomb_data (dcs.hh:604-622).
+int ref_coulomb_data(const double* K, int64_t n, double A, double I, int32_t Z, double mass, double* fCM,
+                     double* screen, double* fspin, double* invlambda) {
+    const noa::pms::AtomicElement el{A, I, Z};
+    dcs::coulomb_data(wrap_shape(fCM, {n, 2}), wrap_shape(screen, {n, 9}), wrap(fspin, n), wrap(invlambda, n),
+                      wrap(K, n), el, mass);
+    return 0;
+}
+
+// dcs::coulomb_transport (dcs.hh:676-694); mu has n_mu = 1 or n values.
+int ref_coulomb_transport(const double* screen, const double* fspin, const double* mu, int64_t n_mu, int64_t n,
+                          double* G) {
+    dcs::coulomb_transport(wrap_shape(G, {n, 2}), wrap_shape(screen, {n, 9}), wrap(fspin, n),
+                           n_mu == 1 ? wrap_shape(mu, {}) : wrap(mu, n_mu));
+    return 0;
+}
+
+// dcs::hard_scattering (dcs.hh:842-872) over [nel][nkin] arrays.
+int ref_hard_scattering(int64_t nel, int64_t nkin, const double* G, const double* fCM, const double* screen,
+                        const double* invlambda, const double* fspin, double* mu0, double* lb_h) {
+    dcs::hard_scattering(wrap(mu0, nkin), wrap(lb_h, nkin), wrap_shape(G, {nel, nkin, 2}),
+                         wrap_shape(fCM, {nel, nkin, 2}), wrap_shape(screen, {nel, nkin, 9}),
+                         wrap_shape(invlambda, {nel, nkin}), wrap_shape(fspin, {nel, nkin}));
+    return 0;
+}
+
+// dcs::soft_scattering (dcs.hh:940-950).
+int ref_soft_scattering(const double* K, int64_t n, double A, double I, int32_t Z, double mass, double* result) {
+    const noa::pms::AtomicElement el{A, I, Z};
+    dcs::soft_scattering(wrap(result, n), wrap(K, n), el, mass);
     return 0;
 }
 

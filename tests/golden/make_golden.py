@@ -1,6 +1,7 @@
-"""Writes tests/golden/dcs_golden.npz from the reference implementation itself.
+"""Writes tests/golden/dcs_golden.npz and tests/golden/coulomb_golden.npz
+from the reference implementation itself.
 
-    make -C oracle ref && python tests/golden/make_golden.py
+    make -C oracle ref && python tests/golden/make_golden.py [dcs|coulomb|all]
 
 oracle/_ref/libnoa_ref.so is the reference's own DCS headers
 (src/noa/pms/dcs.hh, physics.hh, utils/numerics.hh, utils/common.hh) compiled
@@ -17,6 +18,13 @@ outputs (bremsstrahlung, pair production, photonuclear, ionisation) for:
     then float64, q = 0.0505 K; first 64 values).
 Elements: standard rock (Z=11), hydrogen (Z=1 branches), iron, uranium; muon
 mass, plus a tau-mass block for standard rock.
+
+coulomb_golden.npz holds the Coulomb-scattering helpers (dcs.hh:499-950) on a
+K grid from 1e-9 GeV (the K cutoff and the zeta^2 > 1 series) to 1e7 GeV:
+dcs::coulomb_data, dcs::coulomb_transport at mu = 1 (as in
+test/unit/test-dcs-calc.cc:134-166) and at per-K mu from 1e-14 to 1 (both
+transport branches), dcs::hard_scattering for one element and for a
+three-element mixture, and dcs::soft_scattering.
 """
 from __future__ import annotations
 
@@ -79,9 +87,45 @@ def block(el: AtomicElement, mass: float, seed: int) -> dict:
     return {"K": K, "q": q, "dcs": out}
 
 
+COULOMB_K = np.concatenate([[1e-9, 1e-7, 1e-5, 1e-4], 10.0 ** np.linspace(-3, 6, 91), [1e7]])
+
+
+def coulomb_main() -> None:
+    K = COULOMB_K
+    n = K.size
+    mu_k = 10.0 ** np.linspace(-14, 0, n)
+    arrays = {"K": K, "mu_k": mu_k}
+    blocks = {"rock_muon": (STANDARD_ROCK, MUON_MASS), "hydrogen_muon": (ELEMENTS["hydrogen"], MUON_MASS),
+              "uranium_muon": (ELEMENTS["uranium"], MUON_MASS), "rock_tau": (STANDARD_ROCK, TAU_MASS)}
+    per = {}
+    for name, (el, mass) in blocks.items():
+        fCM, scr, fs, il = oracle.coulomb_data(K, el, mass, use_ref=True)
+        G1 = oracle.coulomb_transport(scr, fs, 1.0, use_ref=True)
+        Gk = oracle.coulomb_transport(scr, fs, mu_k, use_ref=True)
+        mu0, lbh = oracle.hard_scattering(G1, fCM, scr, il, fs, use_ref=True)
+        soft = oracle.soft_scattering(K, el, mass, use_ref=True)
+        per[name] = (G1, fCM, scr, il, fs)
+        for k, v in (("fCM", fCM), ("screening", scr), ("fspin", fs), ("invlambda", il), ("G_mu1", G1),
+                     ("G_muk", Gk), ("mu0", mu0), ("lb_h", lbh), ("soft", soft)):
+            arrays[f"{name}_{k}"] = v
+        arrays[f"{name}_element"] = np.array([el.A, el.I, el.Z, mass], dtype=np.float64)
+    mix = ("rock_muon", "hydrogen_muon", "uranium_muon")
+    stacked = [np.stack([per[m][j] for m in mix]) for j in range(5)]
+    mu0, lbh = oracle.hard_scattering(*stacked, use_ref=True)
+    arrays["mixture_mu0"] = mu0
+    arrays["mixture_lb_h"] = lbh
+    np.savez_compressed(os.path.join(HERE, "coulomb_golden.npz"), **arrays)
+    print({k: v.shape for k, v in arrays.items()})
+
+
 def main() -> None:
     if oracle.load_ref() is None:
         raise SystemExit("oracle/_ref/libnoa_ref.so is missing: run `make -C oracle ref` first")
+    which = sys.argv[1] if len(sys.argv) > 1 else "all"
+    if which in ("coulomb", "all"):
+        coulomb_main()
+    if which not in ("dcs", "all"):
+        return
     arrays = {}
     for name, el in ELEMENTS.items():
         b = block(el, MUON_MASS, seed=11)

@@ -1,0 +1,175 @@
+"""Coulomb scattering on the GPU (noa_dcs_coulomb_* / noa_dcs_hard_scattering /
+noa_dcs_soft_scattering through noa_amd.dcs) against the CPU oracle on the
+same inputs, against the reference's own values (the committed fixture), and
+the reference test's flow (test/unit/test-dcs-calc.cc:134-178): coulomb_data,
+coulomb_transport at mu = 1, hard_scattering on [1, nkin, ...] views,
+soft_scattering."""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from noa_amd import _lib, dcs
+from noa_amd.physics import MUON_MASS, STANDARD_ROCK, TAU_MASS, AtomicElement
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = np.load(os.path.join(HERE, "golden", "coulomb_golden.npz"))
+# The GPU log/exp are within 1 ulp of the host libm; the per-K closed forms
+# amplify that by at most a few hundred (pole-reduction cancellations), and
+# the soft-scattering nodes carry the photonuclear DCS tolerance.
+RTOL_CLOSED = 1e-12
+RTOL_SOFT = 1e-10
+
+
+def _rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    d = np.abs(a - b) / np.maximum(np.abs(b), 1e-300)
+    d[(a == 0) & (b == 0)] = 0
+    return float(d.max()) if d.size else 0.0
+
+
+def _gpu_data(K, el, mass, device):
+    Kd = torch.from_numpy(np.ascontiguousarray(K)).to(device)
+    n = Kd.numel()
+    fCM = torch.zeros(n, 2, dtype=torch.float64, device=device)
+    scr = torch.zeros(n, 9, dtype=torch.float64, device=device)
+    fs = torch.zeros(n, dtype=torch.float64, device=device)
+    il = torch.zeros(n, dtype=torch.float64, device=device)
+    dcs.coulomb_data(fCM, scr, fs, il, Kd, el, mass)
+    return Kd, fCM, scr, fs, il
+
+
+@pytest.mark.parametrize("name", ["rock_muon", "hydrogen_muon", "uranium_muon", "rock_tau"])
+def test_coulomb_chain_matches_reference_fixture(name, device):
+    A, I, Z, mass = GOLD[f"{name}_element"]
+    el = AtomicElement(A=float(A), I=float(I), Z=int(Z))
+    K = GOLD["K"]
+    print("coulomb chain", name, K.size, flush=True)
+    Kd, fCM, scr, fs, il = _gpu_data(K, el, float(mass), device)
+    for key, t in (("fCM", fCM), ("screening", scr), ("fspin", fs), ("invlambda", il)):
+        r = _rel(t.cpu().numpy(), GOLD[f"{name}_{key}"])
+        assert r < RTOL_CLOSED, (key, r)
+    G = torch.zeros_like(fCM)
+    dcs.coulomb_transport(G, scr, fs, torch.tensor(1.0, dtype=torch.float64, device=device))
+    assert _rel(G.cpu().numpy(), GOLD[f"{name}_G_mu1"]) < RTOL_CLOSED
+    Gk = torch.zeros_like(fCM)
+    dcs.coulomb_transport(Gk, scr, fs, torch.from_numpy(GOLD["mu_k"]).to(device))
+    assert _rel(Gk.cpu().numpy(), GOLD[f"{name}_G_muk"]) < RTOL_CLOSED
+    n = K.size
+    mu0 = torch.zeros(n, dtype=torch.float64, device=device)
+    lbh = torch.zeros_like(mu0)
+    dcs.hard_scattering(mu0, lbh, G.view(1, n, 2), fCM.view(1, n, 2), scr.view(1, n, 9), il.view(1, n),
+                        fs.view(1, n))
+    assert _rel(mu0.cpu().numpy(), GOLD[f"{name}_mu0"]) < 1e-11   # the reference test's bound
+    assert _rel(lbh.cpu().numpy(), GOLD[f"{name}_lb_h"]) < 1e-11
+    soft = torch.zeros_like(Kd)
+    dcs.soft_scattering(soft, Kd, el, float(mass))
+    assert _rel(soft.cpu().numpy(), GOLD[f"{name}_soft"]) < RTOL_SOFT
+
+
+def test_reference_flow_relative_error(device):
+    """test-dcs-calc.cc:134-178 with the reference's own metric and bounds,
+    the fixture standing in for the unavailable PUMAS tables."""
+    K = GOLD["K"]
+    n = K.size
+    Kd, fCM, scr, fs, il = _gpu_data(K, STANDARD_ROCK, MUON_MASS, device)
+    g = lambda k: torch.from_numpy(GOLD[f"rock_muon_{k}"]).to(device)  # noqa: E731
+    assert dcs.relative_error(scr[:, :3].reshape(-1), g("screening")[:, :3].reshape(-1)).item() < 1e-10
+    assert dcs.mean_error(il, g("invlambda")).item() < 1e-10
+    G = torch.zeros_like(fCM)
+    dcs.coulomb_transport(G, scr, fs, torch.tensor(1.0, dtype=torch.float64, device=device))
+    assert dcs.relative_error(G.view(-1), g("G_mu1").view(-1)).item() < 1e-10
+    mu0 = torch.zeros(n, dtype=torch.float64, device=device)
+    lbh = torch.zeros_like(mu0)
+    dcs.hard_scattering(mu0, lbh, G.view(1, n, 2), fCM.view(1, n, 2), scr.view(1, n, 9), il.view(1, n),
+                        fs.view(1, n))
+    nz = g("mu0") != 0
+    assert dcs.relative_error(mu0[nz], g("mu0")[nz]).item() < 1e-11
+    assert dcs.relative_error(lbh, g("lb_h")).item() < 1e-11
+    soft = torch.zeros_like(Kd)
+    dcs.soft_scattering(soft, Kd, STANDARD_ROCK, MUON_MASS)
+    nz = g("soft") != 0
+    assert dcs.relative_error(soft[nz], g("soft")[nz]).item() < 1e-12
+
+
+def test_fresh_inputs_against_oracle(device):
+    """Seeded K over 1e-6..1e7 GeV, per-K mu over 1e-16..1, a mixture of
+    three elements, and more K than one soft-scattering chunk."""
+    rng = np.random.default_rng(17)
+    n = 20000
+    K = 10.0 ** rng.uniform(-6, 7, n)
+    mu = 10.0 ** rng.uniform(-16, 0, n)
+    els = [(AtomicElement(A=12.0107, I=78e-9, Z=6), MUON_MASS), (STANDARD_ROCK, MUON_MASS),
+           (AtomicElement(A=207.2, I=823e-9, Z=82), MUON_MASS)]
+    per_gpu, per_cpu = [], []
+    for el, mass in els:
+        print("fresh", el.Z, flush=True)
+        Kd, fCM, scr, fs, il = _gpu_data(K, el, mass, device)
+        ref = oracle.coulomb_data(K, el, mass)
+        for t, r in zip((fCM, scr, fs, il), ref):
+            assert _rel(t.cpu().numpy(), r) < RTOL_CLOSED
+        Gk = torch.zeros_like(fCM)
+        dcs.coulomb_transport(Gk, scr, fs, torch.from_numpy(mu).to(device))
+        assert _rel(Gk.cpu().numpy(), oracle.coulomb_transport(ref[1], ref[2], mu)) < RTOL_CLOSED
+        G = torch.zeros_like(fCM)
+        dcs.coulomb_transport(G, scr, fs, torch.tensor([1.0], dtype=torch.float64, device=device))
+        per_gpu.append((G, fCM, scr, il, fs))
+        per_cpu.append((oracle.coulomb_transport(ref[1], ref[2], 1.0), ref[0], ref[1], ref[3], ref[2]))
+    st_g = [torch.stack([p[j] for p in per_gpu]).contiguous() for j in range(5)]
+    st_c = [np.stack([p[j] for p in per_cpu]) for j in range(5)]
+    mu0 = torch.zeros(n, dtype=torch.float64, device=device)
+    lbh = torch.zeros_like(mu0)
+    dcs.hard_scattering(mu0, lbh, *st_g)
+    m0c, lbc = oracle.hard_scattering(*st_c)
+    assert _rel(mu0.cpu().numpy(), m0c) < 1e-11
+    assert _rel(lbh.cpu().numpy(), lbc) < 1e-11
+    # soft scattering over 20000 K (two chunks), against the oracle on a subset
+    el, mass = STANDARD_ROCK, TAU_MASS
+    Kd = torch.from_numpy(K).to(device)
+    soft = torch.zeros_like(Kd)
+    dcs.soft_scattering(soft, Kd, el, mass)
+    idx = np.concatenate([np.arange(0, 300), np.arange(16300, 16500)])
+    want = oracle.soft_scattering(K[idx], el, mass, threads=8)
+    assert _rel(soft.cpu().numpy()[idx], want) < RTOL_SOFT
+
+
+def test_argument_checks(device):
+    K = torch.ones(8, dtype=torch.float64, device=device)
+    with pytest.raises(ValueError):
+        dcs.coulomb_data(torch.zeros(8, 3, dtype=torch.float64, device=device),
+                         torch.zeros(8, 9, dtype=torch.float64, device=device), torch.zeros_like(K),
+                         torch.zeros_like(K), K, STANDARD_ROCK, MUON_MASS)
+    with pytest.raises(ValueError):
+        dcs.coulomb_transport(torch.zeros(8, 2, dtype=torch.float64, device=device),
+                              torch.zeros(8, 9, dtype=torch.float64, device=device), torch.zeros_like(K),
+                              torch.ones(3, dtype=torch.float64, device=device))
+    with pytest.raises(_lib.NoaDcsError):
+        dcs.soft_scattering(torch.zeros_like(K), K, AtomicElement(A=22.0, I=1e-7, Z=0), MUON_MASS)
+    # empty input is a no-op
+    e = torch.zeros(0, dtype=torch.float64, device=device)
+    dcs.soft_scattering(e, e, STANDARD_ROCK, MUON_MASS)
+
+
+def test_libtorch_layer_matches_python_layer(device):
+    """The C++ LibTorch header (include/noa/pms/dcs_hip.hh, through the torch
+    extension) and the Python layer reach the same kernels: equal bit for bit."""
+    from noa_amd import _muon_dcs
+    K = torch.from_numpy(GOLD["K"]).to(device)
+    el = STANDARD_ROCK
+    chain = _muon_dcs.coulomb_chain(K, el.A, el.I, el.Z, MUON_MASS)
+    Kd, fCM, scr, fs, il = _gpu_data(GOLD["K"], el, MUON_MASS, device)
+    for a, b in zip(chain[:4], (fCM, scr, fs, il)):
+        assert torch.equal(a, b)
+    assert _rel(chain[5].cpu().numpy(), GOLD["rock_muon_mu0"]) < 1e-11
+    soft = torch.zeros_like(Kd)
+    dcs.soft_scattering(soft, Kd, el, MUON_MASS)
+    assert torch.equal(_muon_dcs.soft_scattering(K, el.A, el.I, el.Z, MUON_MASS), soft)
+    for cel, integrand in ((False, dcs.del_integrand), (True, dcs.cel_integrand)):
+        r = torch.zeros_like(Kd)
+        dcs.vmap_integral(dcs.recoil_integral(dcs.photonuclear, integrand))(r, Kd, 0.05, el, MUON_MASS, 180)
+        assert torch.equal(_muon_dcs.recoil_integral(2, cel, K, 0.05, el.A, el.I, el.Z, MUON_MASS, 180), r)

@@ -77,7 +77,7 @@ def test_training_reduces_loss(device):
     K, q = S.make_pairs(off, seed=3, device=device)
     ex = S.exact_dcs(K, q, off, els)
     X = S.features(K, q, off, els)
-    T, M, stats = S.targets(ex, els, off)
+    T, M, stats = S.targets(ex, els, off, X)
     net = S.Network(seed=2, max_rows=2048, device=device)
     losses = net.train(X, T, M, epochs=30, batch=2048)
     print("losses", losses[[0, 9, 19, 29]], flush=True)
@@ -91,7 +91,7 @@ def test_flagged_and_rechecked_queries_are_exact(device):
     K, q = S.make_pairs(off, seed=5, device=device)
     ex = S.exact_dcs(K, q, off, els)
     net = S.Network(seed=4, device=device)
-    stats = {"mean": np.zeros(4), "std": np.ones(4)}
+    stats = {"mean": np.zeros(4), "std": np.ones(4), "trend": np.zeros((4, S.NTREND))}
     sur = S.Surrogate(net, els, stats, np.ones((1, S.MAP_BINS, S.MAP_BINS), np.uint8), 1e-3)
     out, cnt = sur.evaluate(K, q, off, recheck_per_mille=0)
     assert S.Surrogate.read_counters(cnt)[0] == 20000
@@ -119,26 +119,61 @@ def test_flagged_and_rechecked_queries_are_exact(device):
     net.close()
 
 
+def test_trend_epilogue(device):
+    """With every weight zero the network outputs y = 0, so the kernel's FP32
+    epilogue alone gives DCS_p = exp(mean_p + trend_p . phi(x)) / A on the
+    active pairs and 0 elsewhere."""
+    els = [STANDARD_ROCK]
+    off = np.array([0, 50000])
+    K, q = S.make_pairs(off, seed=11, device=device)
+    ex = S.exact_dcs(K, q, off, els)
+    X = S.features(K, q, off, els)
+    rng = np.random.default_rng(5)
+    stats = {"mean": rng.uniform(-8, -2, 4), "std": rng.uniform(0.1, 2, 4),
+             "trend": rng.uniform(-0.5, 0.5, (4, S.NTREND))}
+    net = S.Network(seed=1, device=device)
+    net.set_params(np.zeros(net.nparam, np.float32))
+    sur = S.Surrogate(net, els, stats, None, 1e-3)
+    out, cnt = sur.evaluate(K, q, off, recheck_per_mille=0)
+    assert S.Surrogate.read_counters(cnt)[0] == 0
+    Phi = S.trend_basis(X)
+    for p in range(4):
+        ln = float(stats["mean"][p]) + Phi @ torch.from_numpy(stats["trend"][p]).to(device)
+        want = torch.where(ex[p] > 0, torch.exp(ln) / STANDARD_ROCK.A, torch.zeros_like(ln))
+        # the zero pattern follows the kinematic exits (an exact zero inside
+        # them, every node's screening term clamped, is rare)
+        assert ((out[p] == 0) != (want == 0)).double().mean().item() < 2e-3, p
+        nz = (want > 0) & (out[p] > 0)
+        rel = ((out[p].double()[nz] - want[nz]) / want[nz]).abs().max().item()
+        assert rel < 2e-5, (p, rel)
+    net.close()
+
+
 @pytest.mark.skipif(not os.path.exists(S.CHECKPOINT), reason="no trained checkpoint")
 def test_trained_network_gain(device):
     sur = S.Surrogate.load(device=device)
     off = S.equal_offsets(len(sur.elements), 1 << 22)
     K, q = S.make_pairs(off, seed=77, device=device)
     ex = S.exact_dcs(K, q, off, sur.elements)
-    # 1. raw network, no routing: the trained network against an untrained one
+    # 1. raw network, no routing: the trained network against the fitted
+    # cubic trend alone (a network whose weights are all zero outputs y = 0)
     out, _ = sur.evaluate(K, q, off, recheck_per_mille=0, use_map=False)
     trained = S.max_rel_errors(out, ex)
-    blank = S.Surrogate(S.Network(seed=9, device=device), sur.elements, sur.stats, None, sur.tol)
+    zero = S.Network(seed=9, device=device)
+    zero.set_params(np.zeros(zero.nparam, np.float32))
+    blank = S.Surrogate(zero, sur.elements, sur.stats, None, sur.tol)
     out0, _ = blank.evaluate(K, q, off, recheck_per_mille=0, use_map=False)
-    untrained = S.max_rel_errors(out0, ex)
-    print("trained p99", [r["p99_rel"] for r in trained], "untrained p99", [r["p99_rel"] for r in untrained])
-    for a, b in zip(trained, untrained):
-        assert a["p99_rel"] < 0.05 * b["p99_rel"]
-    # 2. with the validation map: every query within the stated tolerance
+    trend_only = S.max_rel_errors(out0, ex)
+    print("trained p99", [r["p99_rel"] for r in trained], "trend-only p99", [r["p99_rel"] for r in trend_only])
+    for a, b in zip(trained, trend_only):
+        assert a["p99_rel"] < 0.1 * b["p99_rel"]
+    zero.close()
+    # 2. with the validation map: every query within the stated tolerance,
+    # and a share of the queries answered without the exact kernel
     out, cnt = sur.evaluate(K, q, off, recheck_per_mille=10)
     n_exact, recheck = S.Surrogate.read_counters(cnt)
     routed = S.max_rel_errors(out, ex)
     print("routed", routed, "exact fraction", n_exact / K.numel(), "recheck", recheck, flush=True)
     for r in routed:
         assert r["max_rel"] <= sur.tol, r
-    assert n_exact < K.numel()
+    assert n_exact < 0.9 * K.numel()
