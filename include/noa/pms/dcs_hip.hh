@@ -248,9 +248,11 @@ inline void coulomb_data(const CMLorentz& fCM, const ScreeningFactors& screening
 
 // dcs.hh:676-694.
 inline void coulomb_transport(const TransportCoefs& coefficients, const ScreeningFactors& screening,
-                              const FSpins& fspin, const AngularCutoff& mu) {
+                              const FSpins& fspin, const AngularCutoff& mu_in) {
     const int64_t n = fspin.numel();
     const auto dev = fspin.device();
+    // the reference's tests pass a CPU scalar, torch::tensor(1.0)
+    const auto mu = (mu_in.numel() == 1 && mu_in.is_cpu()) ? mu_in.to(dev) : mu_in;
     detail::check_f64(fspin, "fspin", n, dev);
     detail::check_f64(screening, "screening", 9 * n, dev);
     detail::check_f64(coefficients, "coefficients", 2 * n, dev);

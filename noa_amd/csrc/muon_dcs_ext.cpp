@@ -67,7 +67,7 @@ std::vector<torch::Tensor> coulomb_chain(torch::Tensor kinetic_energies, double 
     auto fspin = torch::zeros({n}, opt), invlambda = torch::zeros({n}, opt), G = torch::zeros({n, 2}, opt);
     auto mu0 = torch::zeros({n}, opt), lb_h = torch::zeros({n}, opt);
     dcs::coulomb_data(fCM, screen, fspin, invlambda, kinetic_energies, AtomicElement{A, I, Z}, mass);
-    dcs::coulomb_transport(G, screen, fspin, torch::ones({1}, opt));
+    dcs::coulomb_transport(G, screen, fspin, torch::tensor(1.0, torch::dtype(torch::kDouble)));  // as the reference test
     dcs::hard_scattering(mu0, lb_h, G.view({1, n, 2}), fCM.view({1, n, 2}), screen.view({1, n, 9}),
                          invlambda.view({1, n}), fspin.view({1, n}));
     return {fCM, screen, fspin, invlambda, G, mu0, lb_h};

@@ -291,6 +291,8 @@ def coulomb_transport(coefficients: torch.Tensor, screening: torch.Tensor, fspin
     _check_f64("fspin", fspin, (n,), dev)
     _check_f64("screening", screening, (n, 9), dev)
     _check_f64("coefficients", coefficients, (n, 2), dev)
+    if isinstance(mu, torch.Tensor) and mu.numel() == 1 and mu.device.type == "cpu":
+        mu = mu.to(dev)  # the reference's tests pass torch::tensor(1.0) on the CPU
     _check_f64("mu", mu, None, dev)
     if mu.numel() not in (1, n):
         raise ValueError(f"mu must hold 1 or {n} values (got {mu.numel()})")
