@@ -4,7 +4,11 @@
 // path makes ~52 log and ~24 exp calls per (K, q) pair, so libm dominates the
 // VALU issue budget. These versions handle positive normal finite arguments
 // (log) and |x| < 708 (exp) with ~30 and ~22 instructions plus LDS table
-// reads, at about 0.5 ulp; anything else falls through to libm.
+// reads, at about 0.5 ulp. Special arguments are handled in place, without
+// an inlined libm fallback at every call site: log of 0, negatives, inf and
+// NaN by their IEEE values, of subnormals by scaling by 2^54; exp of NaN
+// propagates, and the range beyond +-708 runs the same path, with ldexp
+// producing the overflow, subnormal and zero results.
 //
 // log x = k ln2 + log(1/invc_i) + log1p(r),  r = z invc_i - 1, |r| < 2^-7,
 //   with x = 2^k z, z in [0.6875, 1.375) and i from the top 7 mantissa bits
@@ -77,12 +81,19 @@ NOA_HD inline double from_bits(uint64_t u) {
 
 NOA_HD inline double log(double x, const Tab& t) {
 #pragma clang fp contract(off)
-    const uint64_t ix = as_bits(x);
-    // positive normal finite: 0x0010... <= ix < 0x7ff0...
-    if (ix - 0x0010000000000000ull >= 0x7fe0000000000000ull) return std::log(x);
+    uint64_t ix = as_bits(x);
+    int kadj = 0;
+    // positive normal finite: 0x0010... <= ix < 0x7ff0...; the rest without libm
+    if (ix - 0x0010000000000000ull >= 0x7fe0000000000000ull) {
+        if (x == 0.0) return -INFINITY;
+        if (!(x > 0.0)) return (x == x) ? __builtin_nan("") : x;  // negative: NaN; NaN propagates
+        if (x == INFINITY) return x;
+        ix = as_bits(x * 0x1p54);  // subnormal: scale into the normal range
+        kadj = -54;
+    }
     const uint64_t tmp = ix - LOG_OFF;
     const int i = int((tmp >> 45) & uint64_t(LOG_N - 1));
-    const int k = int(int64_t(tmp) >> 52);
+    const int k = int(int64_t(tmp) >> 52) + kadj;
     const double z = from_bits(ix - (tmp & 0xfff0000000000000ull));
     const double kd = double(k);
     const double r = std::fma(z, t.invc[i], -1.0);
@@ -104,7 +115,10 @@ NOA_HD inline double log(double x, const Tab& t) {
 
 NOA_HD inline double exp(double x, const Tab& t) {
 #pragma clang fp contract(off)
-    if (!(std::fabs(x) < 708.0)) return std::exp(x);  // overflow, underflow, NaN
+    if (x != x) return x;
+    // beyond +-750 the result is inf or 0 either way; the clamp keeps k in
+    // int range, and ldexp rounds the overflow / subnormal results
+    x = std::fmin(std::fmax(x, -750.0), 750.0);
     const double kd = std::rint(x * INV_LN2_N);
     const int ki = int(kd);
     double r = std::fma(-kd, LN2_N_HI, x);

@@ -76,3 +76,34 @@ def test_matches_libm_and_specials(fm):
     with np.errstate(all="ignore"):
         np.testing.assert_array_equal(fm["exp"](se), np.exp(se))
     assert fm["log"](np.array([1.0]))[0] == 0.0 and fm["exp"](np.array([0.0]))[0] == 1.0
+
+
+def test_log_special_and_subnormal_arguments(fm):
+    """0 -> -inf, negatives -> NaN, inf -> inf, NaN -> NaN; subnormals on the
+    same 0.6-ulp path after a 2^54 scaling."""
+    getcontext().prec = 60
+    sp = fm["log"](np.array([0.0, -0.0, -1.0, -1e-300, np.inf, -np.inf, np.nan]))
+    assert sp[0] == -np.inf and sp[1] == -np.inf
+    assert np.isnan(sp[2]) and np.isnan(sp[3]) and np.isnan(sp[5]) and np.isnan(sp[6])
+    assert sp[4] == np.inf
+    rng = np.random.default_rng(9)
+    x = np.concatenate([10 ** rng.uniform(-323.3, -307.7, 800), [5e-324, 1e-323, 2.2250738585072009e-308]])
+    err = ulp_err(fm["log"](x), [Decimal(float(v)).ln() for v in x])
+    assert err.max() < 0.6, err.max()
+
+
+def test_exp_extreme_arguments(fm):
+    """Beyond +-708 the same path: finite results near overflow within 0.6 ulp,
+    subnormal results within 1 ulp, inf / 0 past the ends, NaN propagates."""
+    getcontext().prec = 60
+    rng = np.random.default_rng(10)
+    x = rng.uniform(708.0, 709.78, 300)
+    err = ulp_err(fm["exp"](x), [Decimal(float(v)).exp() for v in x])
+    assert err.max() < 0.6, err.max()
+    x = rng.uniform(-745.1, -708.4, 300)
+    got = fm["exp"](x)
+    exact = [Decimal(float(v)).exp() for v in x]
+    tiny = Decimal(5e-324)
+    assert max(float(abs(Decimal(float(g)) - e) / tiny) for g, e in zip(got, exact)) <= 1.0
+    sp = fm["exp"](np.array([709.8, 750.0, 1e300, np.inf, -745.2, -750.0, -1e300, -np.inf, np.nan]))
+    assert np.all(sp[:4] == np.inf) and np.all(sp[4:8] == 0.0) and np.isnan(sp[8])
