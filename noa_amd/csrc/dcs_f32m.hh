@@ -39,10 +39,26 @@ static __constant__ float GL9_WF[9] = {0.3302393550012598f, 0.1806481606948574f,
 // a / b through the hardware reciprocal (1 ulp), well inside the 1e-5 budget.
 __device__ __forceinline__ float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 
+// log1p(u), u > -1, through the FP64 table log of 1 + u. The sum is exact in
+// FP64 for |u| >= 2^-29 (a 24-bit u), and below that log1p(u) = u to FP32
+// precision while the sum rounds at 2^-53: about 0.5 FP32 ulp, at a third of
+// the cost of the FP32 libm log1pf.
+__device__ __forceinline__ float log1p_f(float u, const fm::Tab& t) {
+    return float(fm::log(1.0 + double(u), t));
+}
+
+// expm1(s) in FP32: e^s - 1 from the FP64 table exp has an absolute error
+// below 2^-52, under an FP32 ulp of the result for |s| >= 2^-20; smaller |s|
+// take s + s^2/2 (truncation s^3/6).
+__device__ __forceinline__ float expm1_f(float s, const fm::Tab& t) {
+    if (fabsf(s) < 0x1p-20f) return fmaf(0.5f * s, s, s);
+    return float(fm::exp(double(s), t) - 1.0);
+}
+
 // h(u) = (log1p(u) - u) / u^2 for u > 0, without the cancellation of the
 // direct form at small u: alternating series below 1/4 (12 terms, truncation
 // < 1e-8 relative), direct form above (cancellation factor < 8).
-__device__ __forceinline__ float log1p_h(float u) {
+__device__ __forceinline__ float log1p_h(float u, const fm::Tab& t) {
     if (u < 0.25f) {
         float s = -1.0f / 13.0f;
         s = fmaf(s, u, 1.0f / 12.0f);
@@ -58,7 +74,7 @@ __device__ __forceinline__ float log1p_h(float u) {
         s = fmaf(s, u, 1.0f / 2.0f);
         return -s;
     }
-    return fdiv(log1pf(u) - u, u * u);
+    return fdiv(log1p_f(u, t) - u, u * u);
 }
 
 struct F32M {
@@ -124,7 +140,7 @@ struct F32M {
         const float cs = float(c.f_ion_c) * Ef * (fdiv(0.5f, P2f) + br * inv_q * inv_q);
         float Delta = 0.f;
         if (K >= c.i_Kthr) {
-            const float L1 = log1pf(2.f * qf / float(me));
+            const float L1 = log1p_f(2.f * qf / float(me), t);
             Delta = 1.16141E-03f * L1 * (__logf(fdiv(4.f * Ef * float(E - q), m2f)) - L1);
         }
         return cs * (1.f + Delta);
@@ -167,8 +183,10 @@ struct F32M {
 #pragma unroll 1
         for (int j = 0; j < 8; j++) {
             const float s = GL8_XF[j] * tmin;
-            const float eps = __expf(s);
-            const float rho = -expm1f(s);
+            // eps = e^s and rho = 1 - eps from one FP64 table exp
+            const double ed = fm::exp(double(s), t);
+            const float eps = float(ed);
+            const float rho = (fabsf(s) < 0x1p-20f) ? -fmaf(0.5f * s, s, s) : float(1.0 - ed);
             const float rho2 = rho * rho;
             const float rho21 = eps * (1.f + rho);
             const float xi = xi_factor * rho21;
@@ -179,16 +197,16 @@ struct F32M {
             if (xi >= 1E+03f) {
                 Be = 0.5f * xi_i * ((3.f - rho2) + 2.f * beta * opr2);
             } else {
-                const float l = log1pf(xi_i);
+                const float l = log1p_f(xi_i, t);
                 Be = (2.f + rho2) * (1.f + beta) * l + fdiv(rho21 - beta, 1.f + xi) +
-                     (3.f + rho2) * xi_i * log1p_h(xi_i);
+                     (3.f + rho2) * xi_i * log1p_h(xi_i, t);
             }
             const float Ye = fdiv(5.f - rho2 + 4.f * beta * opr2,
                                   2.f * (1.f + 3.f * beta) * __logf(3.f + xi_i) - rho2 - 2.f * beta * (2.f - rho2));
             const float xe = (1.f + xi) * (1.f + Ye);
             const float cLi = fdiv(cL, rho21);
             const float la = __logf(fdiv(AZ13 * sqrtf(xe) * qf, qf + cLi * xe));
-            const float lb = log1pf(cLe * xe);
+            const float lb = log1p_f(cLe * xe, t);
             const float Le = la - 0.5f * lb;
             const float Phi_e = fmaxf(Be * Le, 0.f);
             const float dLe = 3e-7f + 1.2e-7f * (fabsf(la) + 0.5f * fabsf(lb));
@@ -197,8 +215,8 @@ struct F32M {
             if (xi <= 1E-03f) {
                 Bmu = 0.5f * xi * (5.f - rho2 + beta * (3.f + rho2));
             } else {
-                Bmu = opr2 * (1.f + 1.5f * beta) * log1pf(xi) + fdiv(xi * (rho21 - beta), 1.f + xi) -
-                      b2 * rho21 * xi * log1p_h(xi);
+                Bmu = opr2 * (1.f + 1.5f * beta) * log1p_f(xi, t) + fdiv(xi * (rho21 - beta), 1.f + xi) -
+                      b2 * rho21 * xi * log1p_h(xi, t);
             }
             const float Ymu = fdiv(4.f + rho2 + 3.f * beta * opr2,
                                    opr2 * (1.5f + 2.f * beta) * __logf(3.f + xi) + 1.f - 1.5f * rho2);
@@ -265,13 +283,13 @@ struct F32M {
             // Q2 = Q2max e^-s; 2Mq - Q2 = c0 - Q2max expm1(-s) (= W^2 - M^2),
             // with expm1 only where e^-s - 1 would cancel
             const float e = __expf(-s);
-            const float em1 = (s < 0.35f) ? expm1f(-s) : e - 1.f;
+            const float em1 = (s < 0.35f) ? expm1_f(-s, t) : e - 1.f;
             const float Q2 = Q2max * e;
             const float D = c0 - Q2max * em1;
             const float x = Q2 * inv2Mq;
             const float omx = D * inv2Mq;
             // ALLM97 F2 (dcs.hh:261-307)
-            const float tl = log1pf(fdiv(log1pf(fdiv(Q2, float(ALLM_Q02))), L0));
+            const float tl = log1p_f(fdiv(log1p_f(fdiv(Q2, float(ALLM_Q02)), t), L0), t);
             const float lnt = __logf(tl);
             const float cP = 0.28067f + (0.28067f - 0.22291f) * (fdiv(1.f, 1.f + __expf(2.1979f * lnt)) - 1.f);
             const float aP = -0.0808f + (-0.0808f + 0.44812f) * (fdiv(1.f, 1.f + __expf(1.1709f * lnt)) - 1.f);

@@ -1,9 +1,21 @@
 // FP64 device physics of the four DCS processes.
 //
-// Every expression keeps the reference's evaluation order and FMA
-// contraction is off in every function body, so the only difference from the
-// reference's CPU path is the device log/exp (the table-driven ones of
-// fastmath.hh, ~0.5 ulp; ocml for log10 and sqrt) against glibc.
+// FMA contraction is off in every function body and the closed forms keep
+// the reference's evaluation order. The node loops of the two integrals are
+// rewritten algebraically to spend fewer divisions and logarithms (the same
+// mathematics with a few ulps of different rounding, far inside the 1e-10
+// parity bound):
+//   pair production  the two screening logarithms of L_e become one log of
+//                    A Z^-1/3 q rho21 sqrt(x_e / ((1 + c_Le x_e)
+//                    (q rho21 + c_L x_e)^2)); c_L / rho21 is folded into the
+//                    log arguments; (rho21 - beta) / (1 + xi) is shared by
+//                    B_e and B_mu; / r^2 is a product with 1 / r^2;
+//   photonuclear     per-pair reciprocals of M q, E^2 and q; W^2 - M^2 =
+//                    2 M q - Q^2; one reciprocal for x_P and x_R, one for the
+//                    two ALLM logistic factors, one for the three terms of
+//                    Whitlow's R, one 1 / Q^2; log10 as log / ln 10.
+// Otherwise the device log/exp (the table-driven ones of fastmath.hh, ~0.5
+// ulp) against glibc is the only difference from the reference's CPU path.
 // Launch invariants come precomputed in DcsConsts.
 //   bremsstrahlung   src/noa/pms/physics.hh:114-153
 //   pair production  src/noa/pms/dcs.hh:144-258 (8-node rule numerics.hh:110-129)
@@ -24,6 +36,8 @@ namespace noa_hip {
 // Gauss-Legendre node tables; the loop index is wave-uniform, so these are
 // scalar loads. Values: numerics.hh:116-121 (8 nodes on [0,1], truncated
 // literals kept as specified) and numerics.hh:137-144 (9 nodes on [-1,1]).
+constexpr double INV_LN10 = 0.43429448190325182765;  // 1 / ln 10
+
 static __constant__ double GL8_X[8] = {0.01985507, 0.10166676, 0.2372338, 0.40828268,
                                 0.59171732, 0.7627662, 0.89833324, 0.98014493};
 static __constant__ double GL8_W[8] = {0.05061427, 0.11119052, 0.15685332, 0.18134189,
@@ -117,17 +131,19 @@ struct F64 {
             const double rho21 = eps * (2. - eps);
             const double xi = xi_factor * rho21;
             const double xi_i = 1. / xi;
+            const double dr = (rho21 - beta) / (1. + xi);  // shared by B_e and B_mu
             double Be;
             if (xi >= 1E+03)
                 Be = 0.5 * xi_i * ((3. - rho2) + 2. * beta * (1. + rho2));
             else
-                Be = ((2. + rho2) * (1. + beta) + xi * (3. + rho2)) * fm::log(1. + xi_i, t) +
-                     (rho21 - beta) / (1. + xi) - 3. - rho2;
+                Be = ((2. + rho2) * (1. + beta) + xi * (3. + rho2)) * fm::log(1. + xi_i, t) + dr - 3. - rho2;
             const double Ye = (5. - rho2 + 4. * beta * (1. + rho2)) /
                               (2. * (1. + 3. * beta) * fm::log(3. + xi_i, t) - rho2 - 2. * beta * (2. - rho2));
             const double xe = (1. + xi) * (1. + Ye);
-            const double cLi = c.p_cL / rho21;
-            const double Le = fm::log(c.p_AZ13 * sqrt(xe) * q / (q + cLi * xe), t) - 0.5 * fm::log(1. + c.p_cLe * xe, t);
+            // q + (c_L / rho21) x = (q rho21 + c_L x) / rho21
+            const double qr = q * rho21;
+            const double ge = qr + c.p_cL * xe;
+            const double Le = fm::log(c.p_AZ13 * qr * sqrt(xe / ((1. + c.p_cLe * xe) * (ge * ge))), t);
             double Phi_e = Be * Le;
             if (Phi_e < 0.) Phi_e = 0.;
             double Bmu;
@@ -135,14 +151,14 @@ struct F64 {
                 Bmu = 0.5 * xi * (5. - rho2 + beta * (3. + rho2));
             else
                 Bmu = ((1. + rho2) * (1. + 1.5 * beta) - xi_i * (1. + 2. * beta) * rho21) * fm::log(1. + xi, t) +
-                      xi * (rho21 - beta) / (1. + xi) + (1. + 2. * beta) * rho21;
+                      xi * dr + (1. + 2. * beta) * rho21;
             const double Ymu = (4. + rho2 + 3. * beta * (1. + rho2)) /
                                ((1. + rho2) * (1.5 + 2. * beta) * fm::log(3. + xi, t) + 1. - 1.5 * rho2);
             const double xmu = (1. + xi) * (1. + Ymu);
-            const double Lmu = fm::log(c.p_rAZ13 * q / (c.p_15Z13 * (q + cLi * xmu)), t);
+            const double Lmu = fm::log(c.p_rAZ13 * qr / (c.p_15Z13 * (qr + c.p_cL * xmu)), t);
             double Phi_mu = Bmu * Lmu;
             if (Phi_mu < 0.) Phi_mu = 0.;
-            const double f = -(Phi_e + Phi_mu / c.p_r2) * (1. - rho) * tmin;
+            const double f = -(Phi_e + Phi_mu * c.p_inv_r2) * (1. - rho) * tmin;
             I += f * 1. * GL8_W[j];
         }
 
@@ -189,22 +205,31 @@ struct F64 {
         const double hd = 0.5 * dpQ2;
         const double Mq = M * q;
         const double EE = E * E;
+        const double twoMq = 2. * Mq;
+        const double inv_2Mq = 0.5 / Mq;
+        const double inv_EE = 1. / EE;
+        const double inv_q = 1. / q;
+        const double inv_l2 = 1. / ALLM_LAMBDA2;
 
         double ds = 0.;
 #pragma unroll 1
         for (int j = 0; j < 9; j++) {
             const double Q2 = fm::exp(pQ2c + hd * GL9_X[j], t);
             // dcs_photonuclear_d2 (dcs.hh:335-355)
-            const double x = 0.5 * Q2 / Mq;
-            // dcs_photonuclear_f2_allm (dcs.hh:261-307)
-            const double M2 = 0.8803505929;
-            const double W2 = M2 + Q2 * (1.0 / x - 1.0);
-            const double tl = fm::log(fm::log((Q2 + ALLM_Q02) / ALLM_LAMBDA2, t) / c.ph_L0, t);
-            const double xP = (Q2 + 49.457) / (Q2 + 49.457 + W2 - M2);
-            const double xR = (Q2 + 0.15052) / (Q2 + 0.15052 + W2 - M2);
+            const double x = Q2 * inv_2Mq;
+            // dcs_photonuclear_f2_allm (dcs.hh:261-307); W^2 - M^2 = 2 M q - Q^2
+            const double D = twoMq - Q2;
+            const double tl = fm::log(fm::log((Q2 + ALLM_Q02) * inv_l2, t) * c.ph_inv_L0, t);
+            const double nP = Q2 + 49.457, nR = Q2 + 0.15052;
+            const double dP = nP + D, dR = nR + D;
+            const double rPR = 1.0 / (dP * dR);
+            const double xP = nP * dR * rPR;
+            const double xR = nR * dP * rPR;
             const double lnt = fm::log(tl, t);
-            const double cP = 0.28067 + (0.28067 - 0.22291) * (1.0 / (1.0 + fm::exp(2.1979 * lnt, t)) - 1.0);
-            const double aP = -0.0808 + (-0.0808 - -0.44812) * (1.0 / (1.0 + fm::exp(1.1709 * lnt, t)) - 1.0);
+            const double gP = 1.0 + fm::exp(2.1979 * lnt, t), gA = 1.0 + fm::exp(1.1709 * lnt, t);
+            const double rG = 1.0 / (gP * gA);
+            const double cP = 0.28067 + (0.28067 - 0.22291) * (gA * rG - 1.0);
+            const double aP = -0.0808 + (-0.0808 - -0.44812) * (gP * rG - 1.0);
             const double bP = 0.36292 + 1.8917 * fm::exp(1.8439 * lnt, t);
             const double cR = 0.80107 + 0.97307 * fm::exp(3.4942 * lnt, t);
             const double aR = 0.58400 + 0.37888 * fm::exp(2.6063 * lnt, t);
@@ -218,17 +243,21 @@ struct F64 {
             if (x < 0.0014)
                 a = c.ph_a_low;
             else if (x < 0.04)
-                a = fm::exp((0.069 * log10(x) + 0.097) * c.ph_logA, t);
+                a = fm::exp((0.069 * (fm::log(x, t) * INV_LN10) + 0.097) * c.ph_logA, t);
             const double F2A = c.ph_halfA * a * (2.0 + x * (-1.85 + x * (2.45 + x * (-2.35 + x)))) * F2p;
-            // dcs_photonuclear_r_whitlow (dcs.hh:322-332)
+            // dcs_photonuclear_r_whitlow (dcs.hh:322-332), over one reciprocal
             double q2 = Q2;
             if (Q2 < 0.3) q2 = 0.3;
-            const double theta = 1. + 12.0 * q2 / (1.0 + q2) * 0.015625 / (0.015625 + x * x);
-            const double R = 0.635 / fm::log(q2 / 0.04, t) * theta + 0.5747 / q2 - 0.3534 / (0.09 + q2 * q2);
+            const double theta = 1. + (12.0 * 0.015625) * q2 / ((1.0 + q2) * (0.015625 + x * x));
+            const double lg = fm::log(q2 * 25.0, t);  // q2 / 0.04
+            const double den = 0.09 + q2 * q2;
+            const double rR = 1.0 / (lg * q2 * den);
+            const double R = (0.635 * theta) * (q2 * den) * rR + 0.5747 * (lg * den) * rR - 0.3534 * (lg * q2) * rR;
+            const double inv_Q2 = 1.0 / Q2;
             const double dds =
-                (1. - y + 0.5 * (1. - c.ph_2m2 / Q2) * (y * y + Q2 / EE) / (1. + R)) / (Q2 * Q2) -
-                0.25 / (EE * Q2);
-            const double d2 = 2.603096E-35 * F2A * dds / q;
+                (1. - y + 0.5 * (1. - c.ph_2m2 * inv_Q2) * (y * y + Q2 * inv_EE) / (1. + R)) * (inv_Q2 * inv_Q2) -
+                0.25 * inv_EE * inv_Q2;
+            const double d2 = 2.603096E-35 * F2A * dds * inv_q;
             ds += d2 * Q2 * 1. * GL9_W[j];
         }
         return (ds < 0.) ? 0. : 0.5 * ds * dpQ2 * 1E+03 * AVOGADRO_NUMBER * (c.mass + K) / c.A;
