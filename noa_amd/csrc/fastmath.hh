@@ -67,6 +67,36 @@ NOA_HD inline double table_entry(int k) {
     return EXP_T_LO[k - 3 * LOG_N - EXP_N];
 }
 
+// a * b + c for a compile-time constant c (fma_kc) or b (fma_kb). gfx950's
+// v_fma_f64 takes one scalar operand, but the compiler shrinks an FMA whose
+// addend is a constant to the two-address v_fmac_f64 and first copies the
+// constant into the destination VGPR with two v_mov_b32, three VALU slots for
+// one FMA. Here the constant is pinned to an SGPR pair (set by s_mov_b32 on
+// the scalar unit), so each polynomial step is one VALU instruction. The
+// constraint "s" makes the operand wave-uniform: pass only constants.
+#if defined(__HIP_DEVICE_COMPILE__)
+__device__ __forceinline__ double fma_kc(double a, double b, double c) {
+    double d;
+    asm("v_fma_f64 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "s"(c));
+    return d;
+}
+__device__ __forceinline__ double fma_kb(double a, double b, double c) {
+    double d;
+    asm("v_fma_f64 %0, %1, %2, %3" : "=v"(d) : "v"(a), "s"(b), "v"(c));
+    return d;
+}
+// -a * b + c with constant b
+__device__ __forceinline__ double fnma_kb(double a, double b, double c) {
+    double d;
+    asm("v_fma_f64 %0, -%1, %2, %3" : "=v"(d) : "v"(a), "s"(b), "v"(c));
+    return d;
+}
+#else
+inline double fma_kc(double a, double b, double c) { return std::fma(a, b, c); }
+inline double fma_kb(double a, double b, double c) { return std::fma(a, b, c); }
+inline double fnma_kb(double a, double b, double c) { return std::fma(-a, b, c); }
+#endif
+
 NOA_HD inline uint64_t as_bits(double x) {
     uint64_t u;
     std::memcpy(&u, &x, sizeof(u));
@@ -97,18 +127,18 @@ NOA_HD inline double log(double x, const Tab& t) {
     const double z = from_bits(ix - (tmp & 0xfff0000000000000ull));
     const double kd = double(k);
     const double r = std::fma(z, t.invc[i], -1.0);
-    const double w = std::fma(kd, LN2_HI, t.lhi[i]);  // exact
-    const double s = w + r;                           // TwoSum(w, r)
+    const double w = fma_kb(kd, LN2_HI, t.lhi[i]);  // exact
+    const double s = w + r;                         // TwoSum(w, r)
     const double bb = s - w;
     const double err = (w - (s - bb)) + (r - bb);
-    double q = std::fma(r, 1.0 / 9.0, -1.0 / 8.0);
-    q = std::fma(q, r, 1.0 / 7.0);
-    q = std::fma(q, r, -1.0 / 6.0);
-    q = std::fma(q, r, 1.0 / 5.0);
-    q = std::fma(q, r, -1.0 / 4.0);
-    q = std::fma(q, r, 1.0 / 3.0);
+    double q = fma_kc(r, 1.0 / 9.0, -1.0 / 8.0);
+    q = fma_kc(q, r, 1.0 / 7.0);
+    q = fma_kc(q, r, -1.0 / 6.0);
+    q = fma_kc(q, r, 1.0 / 5.0);
+    q = fma_kc(q, r, -1.0 / 4.0);
+    q = fma_kc(q, r, 1.0 / 3.0);
     q = std::fma(q, r, -0.5);
-    double lo = std::fma(kd, LN2_LO, t.llo[i]) + err;
+    double lo = fma_kb(kd, LN2_LO, t.llo[i]) + err;
     lo = std::fma(r * r, q, lo);
     return s + lo;
 }
@@ -121,12 +151,12 @@ NOA_HD inline double exp(double x, const Tab& t) {
     x = std::fmin(std::fmax(x, -750.0), 750.0);
     const double kd = std::rint(x * INV_LN2_N);
     const int ki = int(kd);
-    double r = std::fma(-kd, LN2_N_HI, x);
-    r = std::fma(-kd, LN2_N_LO, r);
+    double r = fnma_kb(kd, LN2_N_HI, x);
+    r = fnma_kb(kd, LN2_N_LO, r);
     const int j = ki & (EXP_N - 1);
     const int m = ki >> 7;  // floor(ki / 128)
-    double p = std::fma(r, 1.0 / 120.0, 1.0 / 24.0);
-    p = std::fma(p, r, 1.0 / 6.0);
+    double p = fma_kc(r, 1.0 / 120.0, 1.0 / 24.0);
+    p = fma_kc(p, r, 1.0 / 6.0);
     p = std::fma(p, r, 0.5);
     p = std::fma(p, r, 1.0);
     p = p * r;
