@@ -2,20 +2,20 @@
 //
 // ocml's FP64 log is ~98 VALU instructions and exp ~42 on gfx950, and the DCS
 // path makes ~52 log and ~24 exp calls per (K, q) pair, so libm dominates the
-// VALU issue budget. These versions handle positive normal finite arguments
-// (log) and |x| < 708 (exp) with ~30 and ~22 instructions plus LDS table
-// reads, at about 0.5 ulp. Special arguments are handled in place, without
-// an inlined libm fallback at every call site: log of 0, negatives, inf and
-// NaN by their IEEE values, of subnormals by scaling by 2^54; exp of NaN
-// propagates, and the range beyond +-708 runs the same path, with ldexp
-// producing the overflow, subnormal and zero results.
+// VALU issue budget. These versions take ~25 (log) and ~20 (exp) VALU
+// instructions plus LDS table reads, within 0.75 ulp (log) and 0.6 ulp (exp).
+// Special arguments are handled in place, without an inlined libm fallback at
+// every call site: log of 0, negatives, inf and NaN by their IEEE values, of
+// subnormals by scaling by 2^54; exp of NaN propagates, and the range beyond
+// +-708 runs the same path, with ldexp producing the overflow, subnormal and
+// zero results.
 //
 // log x = k ln2 + log(1/invc_i) + log1p(r),  r = z invc_i - 1, |r| < 2^-7,
 //   with x = 2^k z, z in [0.6875, 1.375) and i from the top 7 mantissa bits
 //   (invc = 1 on the two subintervals that touch 1, so r = z - 1 exactly and
 //   log x keeps full relative accuracy as x -> 1); k LN2_HI + logc_hi is
-//   exact (both multiples of 2^-43), the sum with r is a TwoSum, and
-//   log1p(r) - r is a degree-9 Taylor polynomial.
+//   exact (both multiples of 2^-43), the small parts are summed before they
+//   are added to it, and log1p(r) - r is a degree-9 Taylor polynomial.
 // exp x = 2^m 2^(j/128) exp(r), k = rint(128 x / ln2) = 128 m + j,
 //   r = x - k ln2/128 (two FMAs), exp(r) - 1 a degree-5 Taylor polynomial.
 //
@@ -128,9 +128,6 @@ NOA_HD inline double log(double x, const Tab& t) {
     const double kd = double(k);
     const double r = std::fma(z, t.invc[i], -1.0);
     const double w = fma_kb(kd, LN2_HI, t.lhi[i]);  // exact
-    const double s = w + r;                         // TwoSum(w, r)
-    const double bb = s - w;
-    const double err = (w - (s - bb)) + (r - bb);
     double q = fma_kc(r, 1.0 / 9.0, -1.0 / 8.0);
     q = fma_kc(q, r, 1.0 / 7.0);
     q = fma_kc(q, r, -1.0 / 6.0);
@@ -138,9 +135,11 @@ NOA_HD inline double log(double x, const Tab& t) {
     q = fma_kc(q, r, -1.0 / 4.0);
     q = fma_kc(q, r, 1.0 / 3.0);
     q = std::fma(q, r, -0.5);
-    double lo = fma_kb(kd, LN2_LO, t.llo[i]) + err;
-    lo = std::fma(r * r, q, lo);
-    return s + lo;
+    // w exact and |r| < 2^-7: summing the small parts first leaves two
+    // roundings, <= 0.75 ulp in all (a TwoSum of w + r would buy 0.15 ulp
+    // for five more additions)
+    const double lo = fma_kb(kd, LN2_LO, t.llo[i]);
+    return w + (r + std::fma(r * r, q, lo));
 }
 
 NOA_HD inline double exp(double x, const Tab& t) {
