@@ -137,6 +137,45 @@ def cpu_baseline(n_total: int, min_seconds: float = 10.0) -> dict:
             "cpu_model": cpu_model(), "host_cpus": os.cpu_count()}
 
 
+def consumers(dev, stream, steps: int) -> dict:
+    """The two consumers of the DCS map built on it: DEL/CEL recoil integrals
+    of the four processes (min_points 180, xlow 0.05, as the reference's
+    tests) and the Coulomb chain with soft scattering, over 2^16 log-uniform
+    kinetic energies in [1e-3, 1e6] GeV of standard rock."""
+    import torch
+
+    from noa_amd import dcs
+    from noa_amd.physics import MUON_MASS, STANDARD_ROCK
+    nk = 1 << 16
+    K = torch.logspace(-3, 6, nk, dtype=torch.float64, device=dev)
+    dels = tuple(torch.empty_like(K) for _ in range(4))
+    cels = tuple(torch.empty_like(K) for _ in range(4))
+    w, _ = timed_launches(lambda: dcs.recoil_integrals(K, 0.05, STANDARD_ROCK, MUON_MASS, 180, dels, cels),
+                          steps, 1, stream)
+    out = {"recoil_integrals_f64": {"kinetic_energies_per_s": nk * steps / w,
+                                    "dcs_evals_per_s": nk * 180 * steps / w,
+                                    "config": "2^16 K, 4 processes x (DEL, CEL), min_points 180 (180 DCS "
+                                              "evaluations per K, all four processes each)"}}
+    fCM = torch.empty(nk, 2, dtype=torch.float64, device=dev)
+    scr = torch.empty(nk, 9, dtype=torch.float64, device=dev)
+    fs, il, mu0, lbh, ms1 = (torch.empty_like(K) for _ in range(5))
+    G = torch.empty_like(fCM)
+    one = torch.ones(1, dtype=torch.float64, device=dev)
+
+    def coulomb():
+        dcs.coulomb_data(fCM, scr, fs, il, K, STANDARD_ROCK, MUON_MASS)
+        dcs.coulomb_transport(G, scr, fs, one)
+        dcs.hard_scattering(mu0, lbh, G.view(1, nk, 2), fCM.view(1, nk, 2), scr.view(1, nk, 9), il.view(1, nk),
+                            fs.view(1, nk))
+    wc, _ = timed_launches(coulomb, steps, 1, stream)
+    ws, _ = timed_launches(lambda: dcs.soft_scattering(ms1, K, STANDARD_ROCK, MUON_MASS), steps, 1, stream)
+    out["coulomb_hard_scattering_f64"] = {"kinetic_energies_per_s": nk * steps / wc,
+                                          "config": "2^16 K: coulomb_data, transport at mu = 1, hard_scattering"}
+    out["soft_scattering_f64"] = {"kinetic_energies_per_s": nk * steps / ws,
+                                  "config": "2^16 K: ionisation closed form + 102-node photonuclear quadrature"}
+    return out
+
+
 def main() -> None:
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -226,6 +265,7 @@ def main() -> None:
             forms["libtorch_map_all_f64"] = {"evals_per_s": n * args.steps / wl, "ms_per_call": kl}
         except ImportError as e:
             forms["libtorch_map_all_f64"] = {"skipped": str(e)}
+        forms.update(consumers(dev, stream, max(3, args.steps // 10)))
         try:
             from noa_amd import surrogate
             forms["surrogate"] = surrogate.bench_entry(K, q, ref, args.steps, args.warmup, stream, wall)
