@@ -28,7 +28,7 @@ namespace noa_hip {
 static __constant__ double GL6_X[6] = {0.03376524, 0.16939531, 0.38069041, 0.61930959, 0.83060469, 0.96623476};
 static __constant__ double GL6_W[6] = {0.08566225, 0.18038079, 0.23395697, 0.23395697, 0.18038079, 0.08566225};
 
-constexpr int RECOIL_CHUNK = 16384;  // K values per pass
+constexpr int RECOIL_CHUNK = 65536;  // K values per pass
 
 struct RecoilGrid {
     double xlow;
@@ -53,10 +53,12 @@ __global__ __launch_bounds__(256) void recoil_nodes_kernel(const double* __restr
     __syncthreads();
     const fm::Tab tab = fm::tab_from_block(fm_block);
     const int64_t total = nK * g.nodes;
+    // node-major: node j of K index i at j * nK + i, so that the per-K pass
+    // reads each node plane coalesced
     for (int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; idx < total;
          idx += int64_t(gridDim.x) * blockDim.x) {
-        const int64_t i = idx / g.nodes;
-        const int j = int(idx % g.nodes);
+        const int j = int(idx / nK);
+        const int64_t i = idx - int64_t(j) * nK;
         const double k = K[i];
         double h;
         const double tj = node_t(k, g, j, tab, &h);
@@ -113,14 +115,15 @@ __global__ __launch_bounds__(256) void recoil_reduce_kernel(const double* __rest
                 if (dst_cel[p]) dst_cel[p][i] = analytic_ionisation(k, g.xlow, c, true, tab);
                 continue;
             }
-            const double* d = planes[p] + i * g.nodes;
-            const double* qq = qn + i * g.nodes;
+            const double* d = planes[p] + i;
+            const double* qq = qn + i;
             double sd = 0., sc = 0.;
             for (int j = 0; j < g.nodes; j++) {
                 const double w = GL6_W[j % 6];
-                const double dq = d[j] * qq[j];
+                const double qj = qq[int64_t(j) * nK];
+                const double dq = d[int64_t(j) * nK] * qj;
                 sd += dq * h * w;
-                sc += dq * qq[j] * h * w;
+                sc += dq * qj * h * w;
             }
             if (dst_del[p]) dst_del[p][i] = sd / E;
             if (dst_cel[p]) dst_cel[p][i] = sc / E;

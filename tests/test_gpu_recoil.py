@@ -31,8 +31,8 @@ def test_recoil_integrals_parity(device, el, mass, min_points):
 
 
 def test_reference_call_shape_and_chunking(device):
-    # more K values than one pass (16384) exercises the chunk loop
-    Kh = 10.0 ** np.random.default_rng(2).uniform(-3, 6, 40000)
+    # more K values than one pass (65536) exercises the chunk loop
+    Kh = 10.0 ** np.random.default_rng(2).uniform(-3, 6, 70000)
     K = torch.from_numpy(Kh).to(device)
     res = torch.empty_like(K)
     dcs.vmap_integral(dcs.recoil_integral(dcs.pair_production, dcs.del_integrand))(
