@@ -113,7 +113,7 @@ def test_recoil_argument_errors():
     o = _lib.OutputsF64(ctypes.c_void_p(16), None, None, None)
     f = lib.noa_dcs_recoil_integrals_f64
     assert lib.noa_dcs_recoil_workspace_bytes(100, 180) == 100 * 180 * 48
-    assert lib.noa_dcs_recoil_workspace_bytes(10 ** 6, 180) == 16384 * 180 * 48
+    assert lib.noa_dcs_recoil_workspace_bytes(10 ** 6, 180) == 65536 * 180 * 48
     assert f(ctypes.c_void_p(16), 10, 0.05, ctypes.byref(rock), 0.1, 180, ctypes.byref(o), None, None, 0,
              None) == _lib.NOA_DCS_EINVAL and "workspace" in _lib.last_error()
     assert f(ctypes.c_void_p(16), 10, 1.0, ctypes.byref(rock), 0.1, 180, ctypes.byref(o), None, None, 0,
