@@ -65,8 +65,10 @@ const char* noa_dcs_last_error(void);
 const char* noa_dcs_version(void);
 
 /*
- * FP64 evaluation, bit-for-bit the reference arithmetic up to device libm.
- * Replaces the CPU adaptors dcs::vmap(f) / dcs::pvmap(f)
+ * FP64 evaluation of the reference arithmetic (within 1e-10 relative of the
+ * reference's CPU path, same zero pattern: the closed forms and exits keep
+ * its operation order, the quadrature node loops are rearranged to spend
+ * fewer divisions and logarithms). Replaces the CPU adaptors dcs::vmap(f) / dcs::pvmap(f)
  * (src/noa/pms/dcs.hh:35-48, 62-75) for f in {bremsstrahlung,
  * pair_production, photonuclear, ionisation} (dcs.hh:134-443) and the CUDA
  * dcs::cuda::vmap_bremsstrahlung (src/noa/pms/dcs.cuh:30-41): one call
@@ -201,21 +203,23 @@ typedef struct noa_dcs_surrogate_cfg {
     float trend[4][13];     /*   cubic basis phi(x) = (a, b, a^2, b^2, ab, a^3, b^3, a^2 b, a b^2, z, za, zb,
                                  z^2) of the features x = (a, b, z) */
     const uint8_t* flag_map; /* device [n_elements][bins][bins] over (log10 K in [-3,6], log10 nu in [-6,0]);
-                                non-zero = evaluate that cell exactly; may be NULL */
+                                bit p set = evaluate process p exactly in that cell; may be NULL */
     int32_t bins;
     int32_t recheck_per_mille; /* seeded sample evaluated exactly and compared (10 = 1%) */
     uint64_t seed;
 } noa_dcs_surrogate_cfg;
 
 /* Surrogate DCS of the queries of an element batch (layout as in
- * noa_dcs_batch_create), FP32 outputs. Queries outside the training domain,
- * in flagged cells or in the recheck sample are evaluated by the exact FP64
- * physics instead. fallback_idx (int64) and fallback_el (uint32) are device
- * workspaces of offsets[n_elements] - offsets[0] entries; counters is a
- * device array of 4 uint32: on completion counters[1] = number of queries
- * evaluated exactly, counters[2] = bits of the largest relative error of the
- * surrogate over the recheck sample (FP32). Asynchronous on `stream`; calls
- * on one network are ordered by the stream. */
+ * noa_dcs_batch_create), FP32 outputs. Queries outside the training domain
+ * and the seeded recheck sample take every process from the exact FP64
+ * physics, other queries the processes flagged in their cell. fallback_idx
+ * (int64) and fallback_el (uint32) are device workspaces of 2 x
+ * (offsets[n_elements] - offsets[0]) entries; counters is a device array of
+ * 4 uint32: on completion counters[1] and counters[3] = number of
+ * pair-production and photonuclear evaluations done exactly, counters[2] =
+ * bits of the largest relative error of the network over the recheck sample
+ * (FP32, processes the network serves in that cell). Asynchronous on
+ * `stream`; calls on one network are ordered by the stream. */
 int noa_dcs_surrogate_eval(noa_dcs_mlp* mlp, const noa_dcs_element* elements, const int64_t* offsets,
                            int32_t n_elements, double mass, const double* K, const double* q,
                            const noa_dcs_outputs_f32* out, const noa_dcs_surrogate_cfg* cfg, int64_t* fallback_idx,

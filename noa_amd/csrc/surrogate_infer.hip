@@ -233,7 +233,7 @@ struct SurCfg {
     uint64_t seed;
 };
 
-constexpr uint32_t FB_RECHECK = 0x80000000u;  // fallback entry from the recheck sample only
+constexpr uint32_t FB_RECHECK = 0x80000000u;  // the network's value is compared before the overwrite
 
 __device__ __forceinline__ uint64_t mix64(uint64_t z) {
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
@@ -250,19 +250,51 @@ __device__ __forceinline__ bool ion_active(double K, double q, const DcsConsts& 
     return !((Wmax < X_FRACTION * K) || (q > Wmax)) && !(q <= c.i_Wmin);
 }
 
-// counters[0]: work-item counter, [1]: fallback entries, [2]: recheck max rel error (float bits).
+// Appends this lane's entry to a device list when `want`: one global atomic
+// per wave on counters[slot], entries at list[base + rank].
+__device__ __forceinline__ void wave_append(bool want, int64_t g, uint32_t tag, uint32_t* counters, int slot,
+                                            int64_t* __restrict__ idx, uint32_t* __restrict__ el) {
+    const unsigned long long m = __ballot(want);
+    if (!m) return;
+    const uint32_t rank = __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
+    uint32_t base = 0;
+    if (want && rank == 0) base = atomicAdd(&counters[slot], uint32_t(__popcll(m)));
+    const int leader = __ffsll((long long)m) - 1;
+    base = uint32_t(__builtin_amdgcn_readlane(int(base), leader));
+    if (want) {
+        idx[base + rank] = g;
+        el[base + rank] = tag;
+    }
+}
+
+// counters[0]: work-item counter, [1]: pair-production fallback entries (at
+// fb[0, cap)), [2]: recheck max rel error (float bits), [3]: photonuclear
+// fallback entries (at fb[cap, 2 cap)).
+//
+// Lane l owns query qb + l. Its routing comes first: out-of-domain queries
+// and the seeded recheck sample take every process exactly, others the
+// processes whose bit is set in the query's (element, log K, log nu) cell of
+// the flag map. A wave whose queries all need no network value skips the
+// network. Flagged bremsstrahlung and ionisation (closed forms) are
+// evaluated here; flagged active pair production and photonuclear queries
+// are appended to one dense list per process for fallback_kernel, so its
+// waves run one integral each.
 __global__ __launch_bounds__(IWG) void surrogate_dcs_kernel(
     const double* __restrict__ K, const double* __restrict__ q, const SurElem* __restrict__ els,
     const SurItem* __restrict__ items, uint32_t n_items, uint32_t* counters, Outs<float> o, SurCfg cfg,
-    int64_t* __restrict__ fb_idx, uint32_t* __restrict__ fb_el, const uint16_t* __restrict__ frag,
+    int64_t* __restrict__ fb_idx, uint32_t* __restrict__ fb_el, int64_t cap, const uint16_t* __restrict__ frag,
     const float* __restrict__ P) {
     __shared__ MlpLds L;
+    __shared__ double fm_block[fm::TABLE_DOUBLES];
     __shared__ uint32_t s_item;
     load_lds(L, frag, P);
+    for (int k = threadIdx.x; k < fm::TABLE_DOUBLES; k += IWG) fm_block[k] = fm::table_entry(k);
+    const fm::Tab tab = fm::tab_from_block(fm_block);
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    float worst = 0.f;
     for (;;) {
         if (threadIdx.x == 0) s_item = atomicAdd(&counters[0], 1u);
-        __syncthreads();
+        __syncthreads();  // also publishes the LDS contents on the first pass
         const uint32_t it = __builtin_amdgcn_readfirstlane(s_item);
         __syncthreads();  // every wave has read s_item before it is rewritten
         if (it >= n_items) break;  // workgroup-uniform exit
@@ -272,46 +304,55 @@ __global__ __launch_bounds__(IWG) void surrogate_dcs_kernel(
         const int64_t end = w.begin + w.len;
         const int64_t qb = w.begin + int64_t(wave) * QW;
         if (qb >= end) continue;  // wave-uniform: this wave has no queries in this item
-        float x[2][IN];
-#pragma unroll
-        for (int nt = 0; nt < 2; nt++) {
-            const int64_t g = qb + 32 * nt + lane % 32;
-            if (g < end) {
-                features(K[g], q[g], E.c.Z, x[nt]);
-            } else {
-#pragma unroll
-                for (int j = 0; j < IN; j++) x[nt][j] = 0.f;
-            }
-        }
-        // lane l's own query qb + l is N-tile l/32's column l%32
-        float xs[IN];
-#pragma unroll
-        for (int j = 0; j < IN; j++) xs[j] = lane < 32 ? x[0][j] : x[1][j];
-        float out[2][OUT];
-        mlp_wave(L, x, out);
-        // lane l takes query qb + l: N-tile 1's outputs move from lane l-32
-        float t[OUT];
-#pragma unroll
-        for (int p = 0; p < OUT; p++) {
-            const float o1 = __shfl(out[1][p], lane & 31);
-            t[p] = lane < 32 ? out[0][p] : o1;
-        }
         const int64_t g = qb + lane;
-        bool flag = false, recheck_only = false;
-        if (g < end) {
-            const double k = K[g], qq = q[g];
+        const bool live = g < end;
+        double k = 1., qq = 0.5;
+        uint32_t cellmask = 0, mask = 0;
+        bool sample = false;
+        float xs[IN] = {0.f, 0.f, 0.f};
+        if (live) {
+            k = K[g];
+            qq = q[g];
             const double nu = qq / k;
-            const bool ood = !(k >= 1e-3 && k <= 1e6 && nu >= 1e-6 && nu < 1.0);
-            bool cell = false;
-            if (!ood && cfg.flag_map) {
-                const int bins = cfg.bins;
-                const int ik = min(bins - 1, max(0, int((log10(k) + 3.0) / 9.0 * bins)));
-                const int in = min(bins - 1, max(0, int((log10(nu) + 6.0) / 6.0 * bins)));
-                cell = cfg.flag_map[(int64_t(w.element) * bins + ik) * bins + in] != 0;
+            if (!(k >= 1e-3 && k <= 1e6 && nu >= 1e-6 && nu < 1.0)) {
+                mask = 0xFu;  // out of the trained domain
+            } else {
+                const double lk = fm::log(k, tab) * INV_LN10, lnu = fm::log(nu, tab) * INV_LN10;
+                xs[0] = (float(lk) - LOGK_MID) / LOGK_HALF;
+                xs[1] = (float(lnu) - LOGNU_MID) / LOGNU_HALF;
+                xs[2] = float(E.c.Z) / 46.0f - 1.0f;
+                if (cfg.flag_map) {
+                    const int bins = cfg.bins;
+                    const int ik = min(bins - 1, max(0, int((lk + 3.0) / 9.0 * bins)));
+                    const int in = min(bins - 1, max(0, int((lnu + 6.0) / 6.0 * bins)));
+                    cellmask = cfg.flag_map[(int64_t(w.element) * bins + ik) * bins + in] & 0xFu;
+                }
+                sample = (mix64(cfg.seed ^ uint64_t(g)) % 1000u) < cfg.recheck_per_mille;
+                mask = sample ? 0xFu : cellmask;
             }
-            const bool sample = (mix64(cfg.seed ^ uint64_t(g)) % 1000u) < cfg.recheck_per_mille;
-            flag = ood || cell || sample;
-            recheck_only = sample && !ood && !cell;
+        }
+        const bool need_net = live && (mask != 0xFu || sample) && (mask != 0xFu || cellmask != 0xFu);
+        float t[OUT] = {0.f, 0.f, 0.f, 0.f};
+        if (__ballot(need_net)) {  // wave-uniform
+            // MFMA layout: column lane%32 of N-tile nt is query qb + 32 nt + lane%32
+            float x[2][IN];
+#pragma unroll
+            for (int j = 0; j < IN; j++) {
+                x[0][j] = __shfl(xs[j], lane & 31);
+                x[1][j] = __shfl(xs[j], 32 | (lane & 31));
+            }
+            float out[2][OUT];
+            mlp_wave(L, x, out);
+            // lane l takes query qb + l: N-tile 1's outputs move from lane l-32
+#pragma unroll
+            for (int p = 0; p < OUT; p++) {
+                const float o1 = __shfl(out[1][p], lane & 31);
+                t[p] = lane < 32 ? out[0][p] : o1;
+            }
+        }
+        bool to_pair = false, to_photo = false;
+        uint32_t tag_pair = uint32_t(w.element), tag_photo = uint32_t(w.element);
+        if (live) {
             const bool act[OUT] = {true, F64::pair_active(k, qq, E.c), F64::photo_active(k, qq, E.c),
                                    ion_active(k, qq, E.c)};
             const float a = xs[0], b = xs[1], z = xs[2];
@@ -320,59 +361,65 @@ __global__ __launch_bounds__(IWG) void surrogate_dcs_kernel(
             float* dst[OUT] = {o.b, o.p, o.ph, o.i};
 #pragma unroll
             for (int p = 0; p < OUT; p++) {
+                if (!dst[p]) continue;
                 float lnad = fmaf(cfg.stdv[p], t[p], cfg.mean[p]);
 #pragma unroll
                 for (int j = 0; j < NTREND; j++) lnad = fmaf(cfg.trend[p][j], phi[j], lnad);
-                if (dst[p]) dst[p][g] = act[p] ? __expf(lnad - E.ln_a) : 0.f;
+                const float net = __expf(lnad - E.ln_a);
+                const bool exact = (mask >> p) & 1u;
+                const bool recheck = sample && !((cellmask >> p) & 1u);
+                float v = net;
+                if (!act[p]) {
+                    v = 0.f;
+                } else if (exact && (p == 0 || p == 3)) {
+                    const double ex = (p == 0) ? F64::brems(k, qq, E.c, tab) : F64::ion(k, qq, E.c, tab);
+                    if (recheck && ex > 0.) worst = fmaxf(worst, float(fabs(double(net) / ex - 1.)));
+                    v = float(ex);
+                } else if (exact && p == 1) {
+                    to_pair = true;  // the network value stays for the recheck comparison
+                    if (recheck) tag_pair |= FB_RECHECK;
+                } else if (exact && p == 2) {
+                    to_photo = true;
+                    if (recheck) tag_photo |= FB_RECHECK;
+                }
+                dst[p][g] = v;
             }
         }
-        // append flagged queries: one global atomic per wave
-        const unsigned long long m = __ballot(flag);
-        if (m) {
-            const uint32_t rank = __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
-            uint32_t base = 0;
-            if (flag && rank == 0) base = atomicAdd(&counters[1], uint32_t(__popcll(m)));
-            const int leader = __ffsll((long long)m) - 1;
-            base = uint32_t(__builtin_amdgcn_readlane(int(base), leader));
-            if (flag) {
-                fb_idx[base + rank] = g;
-                fb_el[base + rank] = uint32_t(w.element) | (recheck_only ? FB_RECHECK : 0u);
-            }
-        }
+        wave_append(to_pair, g, tag_pair, counters, 1, fb_idx, fb_el);
+        wave_append(to_photo, g, tag_photo, counters, 3, fb_idx + cap, fb_el + cap);
     }
+    // non-negative floats order like their bit patterns
+    if (worst > 0.f) atomicMax(&counters[2], __float_as_uint(worst));
 }
 
-// Exact FP64 evaluation of the fallback entries; for recheck-only entries the
-// surrogate's value (already in the outputs) is compared first.
+// Exact FP64 pair production (entries [0, counters[1])) and photonuclear
+// (entries [cap, cap + counters[3])) of the fallback lists; every entry is
+// kinematically active. For recheck entries the network's value, already in
+// the output, is compared first.
 __global__ __launch_bounds__(256) void fallback_kernel(const double* __restrict__ K, const double* __restrict__ q,
                                                        const SurElem* __restrict__ els,
                                                        const int64_t* __restrict__ fb_idx,
-                                                       const uint32_t* __restrict__ fb_el, uint32_t* counters,
-                                                       Outs<float> o) {
+                                                       const uint32_t* __restrict__ fb_el, int64_t cap,
+                                                       uint32_t* counters, Outs<float> o) {
     __shared__ double fm_block[fm::TABLE_DOUBLES];
     load_fm_tables(fm_block);
     __syncthreads();
     const fm::Tab tab = fm::tab_from_block(fm_block);
-    const uint32_t n = counters[1];
+    const int64_t n_pair = counters[1], n_photo = counters[3];
     float worst = 0.f;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const int64_t g = fb_idx[i];
-        const uint32_t tag = fb_el[i];
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n_pair + n_photo;
+         i += int64_t(gridDim.x) * blockDim.x) {
+        const bool pair = i < n_pair;
+        const int64_t slot = pair ? i : cap + (i - n_pair);
+        const int64_t g = fb_idx[slot];
+        const uint32_t tag = fb_el[slot];
         const DcsConsts& c = els[tag & ~FB_RECHECK].c;
         const double k = K[g], qq = q[g];
-        double v[OUT];
-        v[0] = F64::brems(k, qq, c, tab);
-        v[1] = F64::pair_active(k, qq, c) ? F64::pair(k, qq, c, tab) : 0.;
-        v[2] = F64::photo_active(k, qq, c) ? F64::photo(k, qq, c, tab) : 0.;
-        v[3] = F64::ion(k, qq, c, tab);
-        float* dst[OUT] = {o.b, o.p, o.ph, o.i};
-        for (int p = 0; p < OUT; p++) {
-            if (!dst[p]) continue;
-            if ((tag & FB_RECHECK) && v[p] > 0.) worst = fmaxf(worst, float(fabs(double(dst[p][g]) / v[p] - 1.)));
-            dst[p][g] = float(v[p]);
-        }
+        const double v = pair ? F64::pair(k, qq, c, tab) : F64::photo(k, qq, c, tab);
+        float* dst = pair ? o.p : o.ph;
+        if ((tag & FB_RECHECK) && v > 0.) worst = fmaxf(worst, float(fabs(double(dst[g]) / v - 1.)));
+        dst[g] = float(v);
     }
-    // non-negative floats order like their bit patterns
     if (worst > 0.f) atomicMax(&counters[2], __float_as_uint(worst));
 }
 
@@ -483,10 +530,10 @@ int noa_dcs_surrogate_eval(noa_dcs_mlp* net, const noa_dcs_element* elements, co
     if (!items.empty()) {
         const unsigned grid = unsigned(std::min<int64_t>(int64_t(items.size()), int64_t(n_cus())));
         hipLaunchKernelGGL(surrogate_dcs_kernel, dim3(grid), dim3(IWG), 0, s, K, q, d_els, d_items,
-                           uint32_t(items.size()), counters, o, c, fallback_idx, fallback_el, net->packed,
+                           uint32_t(items.size()), counters, o, c, fallback_idx, fallback_el, total, net->packed,
                            net->params);
-        hipLaunchKernelGGL(fallback_kernel, dim3(unsigned(n_cus()) * 4), dim3(256), 0, s, K, q, d_els, fallback_idx,
-                           fallback_el, counters, o);
+        hipLaunchKernelGGL(fallback_kernel, dim3(unsigned(n_cus()) * 8), dim3(256), 0, s, K, q, d_els, fallback_idx,
+                           fallback_el, total, counters, o);
     }
     if (int rc = check_hip(hipGetLastError(), "surrogate launch")) return rc;
     return ok();

@@ -30,6 +30,8 @@ NPROC = 4
 HERE = os.path.dirname(os.path.abspath(__file__))
 CHECKPOINT = os.path.join(HERE, "surrogate_48el.npz")
 MAP_BINS = 64
+NO_SAMPLES = -1.0  # error-map value of a cell where the process had no non-zero validation sample
+MIN_CELL_SAMPLES = 8  # fewer non-zero samples than this: the cell's worst error is unknown (+inf)
 
 
 def _p(t):
@@ -198,14 +200,33 @@ class Surrogate:
                          if flag_map is not None else None)
 
     def set_tolerance(self, tol: float) -> None:
-        """Re-derive the flag map from the validation error map: a cell is
-        served by the network only where its worst validation error is below
-        half the tolerance."""
+        """Re-derive the flag map from the validation error map: process p of
+        a cell is served by the network only where its worst validation error
+        there is below half the tolerance (bit p of the cell's flag byte set =
+        evaluate exactly). A map of the worst error over the processes
+        ([elements, bins, bins]) flags all four together."""
         if self.cell_err is None:
             raise ValueError("no validation error map to derive flags from")
         self.tol = tol
-        self.flag_map_host = (self.cell_err > 0.5 * tol).astype(np.uint8)
+        bad = (self.cell_err > 0.5 * tol) | (self.cell_err < 0)
+        if bad.ndim == 3:
+            flags = np.where(bad, 0xF, 0)
+        else:
+            flags = sum(bad[p].astype(np.int64) << p for p in range(NPROC))
+        self.flag_map_host = flags.astype(np.uint8)
         self.flag_map = torch.from_numpy(self.flag_map_host).to(self.net.device)
+
+    def served_fraction(self) -> np.ndarray:
+        """Share of cells served by the network, per process, among the cells
+        where the validation saw the process open (all cells without an
+        error map)."""
+        f = np.stack([(self.flag_map_host >> p) & 1 for p in range(NPROC)])
+        res = []
+        for p in range(NPROC):
+            have = (self.cell_err[p] >= 0) if (self.cell_err is not None and self.cell_err.ndim == 4) else \
+                np.ones(f[p].shape, bool)
+            res.append(float(((f[p] == 0) & have).sum()) / max(int(have.sum()), 1))
+        return np.array(res)
 
     def config(self, recheck_per_mille: int = 10, seed: int = 0, use_map: bool = True) -> _lib.SurrogateConfig:
         c = _lib.SurrogateConfig()
@@ -222,7 +243,7 @@ class Surrogate:
 
     def evaluate(self, K, q, offsets, mass=MUON_MASS, recheck_per_mille: int = 10, seed: int = 0,
                  use_map: bool = True, outputs=None, workspace=None):
-        """Surrogate DCS (FP32) with exact fallback; returns (outputs, n_exact, recheck_max_rel)."""
+        """Surrogate DCS (FP32) with exact fallback; returns (outputs, counters) (see read_counters)."""
         n = K.numel()
         dev = K.device
         if outputs is None:
@@ -242,14 +263,23 @@ class Surrogate:
 
     @staticmethod
     def workspace(n: int, dev):
-        return (torch.empty(max(n, 1), dtype=torch.int64, device=dev),
-                torch.empty(max(n, 1), dtype=torch.int32, device=dev),
+        """Fallback lists (pair production and photonuclear, n entries each) and counters."""
+        return (torch.empty(2 * max(n, 1), dtype=torch.int64, device=dev),
+                torch.empty(2 * max(n, 1), dtype=torch.int32, device=dev),
                 torch.zeros(4, dtype=torch.int32, device=dev))
 
     @staticmethod
     def read_counters(counters) -> tuple[int, float]:
+        """(exact integral evaluations: pair production + photonuclear, largest
+        relative error of the network over the recheck sample)."""
         c = counters.cpu().numpy().astype(np.uint32)
-        return int(c[1]), float(np.frombuffer(c[2:3].tobytes(), dtype=np.float32)[0])
+        return int(c[1]) + int(c[3]), float(np.frombuffer(c[2:3].tobytes(), dtype=np.float32)[0])
+
+    @staticmethod
+    def read_counters_full(counters) -> dict:
+        c = counters.cpu().numpy().astype(np.uint32)
+        return {"exact_pair": int(c[1]), "exact_photo": int(c[3]),
+                "recheck_max_rel": float(np.frombuffer(c[2:3].tobytes(), dtype=np.float32)[0])}
 
     def save(self, path: str = CHECKPOINT) -> None:
         np.savez_compressed(path, params=self.net.get_params(), mean=self.stats["mean"], std=self.stats["std"],
@@ -274,8 +304,12 @@ class Surrogate:
 
 
 def error_map(sur_out, exact, K, q, offsets, bins: int = MAP_BINS) -> torch.Tensor:
-    """Worst relative error per (element, log10 K bin, log10 nu bin) over the
-    four processes; a zero / non-zero mismatch counts as error 1."""
+    """Worst relative error per process and (element, log10 K bin, log10 nu
+    bin), [4, elements, bins, bins]; a zero / non-zero mismatch counts as
+    error 1. A cell where the process had no non-zero sample reads
+    NO_SAMPLES (-1); one with fewer than MIN_CELL_SAMPLES reads +inf: the
+    flags send both to the exact path (free for the queries where the
+    process is kinematically closed, which return 0 either way)."""
     dev = K.device
     n = K.numel()
     el = torch.empty(n, dtype=torch.int64, device=dev)
@@ -285,15 +319,18 @@ def error_map(sur_out, exact, K, q, offsets, bins: int = MAP_BINS) -> torch.Tens
     ik = ((torch.log10(K) + 3.0) / 9.0 * bins).long().clamp(0, bins - 1)
     inu = ((torch.log10(nu) + 6.0) / 6.0 * bins).long().clamp(0, bins - 1)
     cell = (el * bins + ik) * bins + inu
-    worst = torch.zeros(n, dtype=torch.float32, device=dev)
+    maps = []
     for p in range(NPROC):
         s = sur_out[p].double()
         e = exact[p]
         rel = torch.where(e > 0, (s - e).abs() / e.clamp_min(1e-300), (s != 0).double())
-        worst = torch.maximum(worst, rel.float())
-    m = torch.zeros((len(offsets) - 1) * bins * bins, dtype=torch.float32, device=dev)
-    m.scatter_reduce_(0, cell, worst, reduce="amax")
-    return m.reshape(len(offsets) - 1, bins, bins)
+        m = torch.zeros((len(offsets) - 1) * bins * bins, dtype=torch.float32, device=dev)
+        m.scatter_reduce_(0, cell, rel.float(), reduce="amax")
+        cnt = torch.zeros_like(m).scatter_add_(0, cell, (e > 0).float())
+        m = torch.where(cnt == 0, torch.full_like(m, NO_SAMPLES),
+                        torch.where(cnt < MIN_CELL_SAMPLES, torch.full_like(m, float("inf")), m))
+        maps.append(m.reshape(len(offsets) - 1, bins, bins))
+    return torch.stack(maps)
 
 
 def max_rel_errors(out, exact) -> list[dict]:
@@ -339,28 +376,43 @@ def train_surrogate(elements, n_train: int = 1 << 22, n_val: int = 1 << 24, epoc
     t_train = time.time() - t1
     log(f"trained {epochs} epochs in {t_train:.1f}s; loss {losses[0]:.3e} -> {losses[-1]:.3e}")
     del K, q, X, T, M
-    # validation on held-out pairs (disjoint seed): error map and flags
-    off_va = equal_offsets(len(elements), n_val)
-    Kv, qv = make_pairs(off_va, seed=2000 + seed, device=dev)
-    exv = exact_dcs(Kv, qv, off_va, elements)
     sur = Surrogate(net, elements, stats, None, tol)
-    outv, cnt = sur.evaluate(Kv, qv, off_va, recheck_per_mille=0, use_map=False)
+    report = {"n_train": n_train, "epochs": epochs, "batch": batch, "train_s": t_train,
+              "loss_first": float(losses[0]), "loss_last": float(losses[-1]), "lr_max": lr_max, "lr_min": lr_min}
+    report.update(validate(sur, n_val=n_val, seed=seed, log=log))
+    return sur, report
+
+
+def validate(sur: Surrogate, n_val: int = 1 << 24, seed: int = 0, log=print) -> dict:
+    """Held-out validation (disjoint seed): the per-process, per-cell worst
+    error map the flags derive from (stored in sur.cell_err, flags set for
+    sur.tol), and the raw error figures."""
+    dev = sur.net.device
+    off_va = equal_offsets(len(sur.elements), n_val)
+    Kv, qv = make_pairs(off_va, seed=2000 + seed, device=dev)
+    exv = exact_dcs(Kv, qv, off_va, sur.elements)
+    outv, _ = sur.evaluate(Kv, qv, off_va, recheck_per_mille=0, use_map=False)
     emap = error_map(outv, exv, Kv, qv, off_va)
     raw = max_rel_errors(outv, exv)
     sur.cell_err = emap.cpu().numpy().astype(np.float32)
-    sur.set_tolerance(tol)
-    flags = sur.flag_map_host
-    report = {"n_train": n_train, "n_val": n_val, "epochs": epochs, "batch": batch, "train_s": t_train,
-              "loss_first": float(losses[0]), "loss_last": float(losses[-1]), "tol": tol,
-              "val_raw_errors": raw, "flagged_cells": float(flags.mean()),
-              "median_cell_err": float(emap.median().item()), "lr_max": lr_max, "lr_min": lr_min,
-              # share of (element, log K, log nu) cells the network serves at other tolerances
-              "cells_within": {str(tl): float((emap <= 0.5 * tl).float().mean().item())
+    sur.set_tolerance(sur.tol)
+    worst = emap.max(dim=0).values
+    report = {"n_val": n_val, "tol": sur.tol, "val_raw_errors": raw,
+              "served_cells_per_process": sur.served_fraction().tolist(),
+              "median_cell_err_per_process": [float(emap[p][emap[p] >= 0].median().item()) for p in range(NPROC)],
+              # share of the cells with samples that the network serves, per
+              # process and for all four, at other tolerances
+              "cells_within": {str(tl): {"per_process": [float(((emap[p] >= 0) & (emap[p] <= 0.5 * tl)).sum().item()
+                                                               / max(int((emap[p] >= 0).sum().item()), 1))
+                                                         for p in range(NPROC)],
+                                         "all_four": float(((worst >= 0) & (worst <= 0.5 * tl)).sum().item()
+                                                           / max(int((worst >= 0).sum().item()), 1))}
                                for tl in (1e-3, 3e-3, 1e-2, 3e-2, 1e-1)}}
-    sur.meta = report
+    sur.meta = {**sur.meta, **report}
     log(f"validation raw max rel err {[round(r['max_rel'], 4) for r in raw]}, p99 "
-        f"{[round(r['p99_rel'], 5) for r in raw]}; flagged cells {flags.mean():.3f}")
-    return sur, report
+        f"{[round(r['p99_rel'], 5) for r in raw]}; served cells per process "
+        f"{[round(x, 3) for x in report['served_cells_per_process']]}")
+    return report
 
 
 def smoke(dev) -> str:
@@ -377,10 +429,11 @@ def smoke(dev) -> str:
     losses = net.train(X, T, M, epochs=2, batch=1024)
     d = (net.forward(X) - net.forward_fused(X)).abs().max().item()
     assert d < 2e-2, d
-    sur = Surrogate(net, els, stats, np.ones((1, MAP_BINS, MAP_BINS), np.uint8), 1e-3)
+    sur = Surrogate(net, els, stats, np.full((1, MAP_BINS, MAP_BINS), 0xF, np.uint8), 1e-3)
     out, cnt = sur.evaluate(K, q, off, recheck_per_mille=0)
     nfb, _ = Surrogate.read_counters(cnt)
-    assert nfb == 8192
+    n_act = int((ex[1] > 0).sum().item() + (ex[2] > 0).sum().item())
+    assert abs(nfb - n_act) <= 16, (nfb, n_act)
     for p in range(NPROC):
         assert torch.equal(out[p], ex[p].float()), p
     net.close()
@@ -426,12 +479,14 @@ def bench_entry(K, q, ref, steps, warmup, stream, wall_f64, tolerances=(None, 3e
             sur.evaluate(Kq, qq, off, outputs=outs, workspace=ws)
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t) / reps
-        n_exact, recheck = Surrogate.read_counters(ws[2])
+        c = Surrogate.read_counters_full(ws[2])
         errs = max_rel_errors(outs, plan.outputs)
         res[f"tol_{tol:g}"] = {
-            "evals_per_s": n / dt, "speedup_vs_f64": dt_exact / dt, "exact_fraction": n_exact / n,
-            "flagged_cells": float(sur.flag_map_host.mean()) if sur.flag_map_host is not None else 0.0,
-            "recheck_max_rel": recheck, "max_rel_err_vs_f64": errs}
+            "evals_per_s": n / dt, "speedup_vs_f64": dt_exact / dt,
+            "exact_pair_fraction": c["exact_pair"] / n, "exact_photo_fraction": c["exact_photo"] / n,
+            "served_cells_per_process": ([round(float(x), 4) for x in sur.served_fraction()]
+                                         if sur.flag_map_host is not None else [1.0] * NPROC),
+            "recheck_max_rel": c["recheck_max_rel"], "max_rel_err_vs_f64": errs}
     plan.close()
     return res
 
@@ -450,7 +505,16 @@ def main() -> None:
     ap.add_argument("--tol", type=float, default=1e-3)
     ap.add_argument("--lr", type=float, default=2e-3, help="peak learning rate of the cosine schedule")
     ap.add_argument("--out", default=CHECKPOINT)
+    ap.add_argument("--revalidate", metavar="CHECKPOINT",
+                    help="recompute the validation error map of a trained checkpoint and write it to --out")
     a = ap.parse_args()
+    if a.revalidate:
+        sur = Surrogate.load(a.revalidate)
+        sur.tol = a.tol
+        report = validate(sur, n_val=a.n_val, log=lambda s: print(s, flush=True))
+        sur.save(a.out)
+        print(json.dumps(report), flush=True)
+        return
     elements, _ = workload.element_batch(48, seed=0)
     sur, report = train_surrogate(elements, n_train=a.n_train, n_val=a.n_val, epochs=a.epochs, batch=a.batch,
                                   tol=a.tol, log=lambda s: print(s, flush=True), lr_max=a.lr, lr_min=a.lr / 100)

@@ -87,37 +87,60 @@ def test_training_reduces_loss(device):
     net.close()
 
 
+def _n_active(ex, sl=slice(None)):
+    return int((ex[1][sl] > 0).sum().item() + (ex[2][sl] > 0).sum().item())
+
+
 def test_flagged_and_rechecked_queries_are_exact(device):
+    """Routing: flagged processes, the recheck sample and out-of-domain
+    queries come from the exact FP64 physics (pair production and
+    photonuclear through the dense fallback lists, one entry per active
+    pair); served processes from the network."""
     els = [STANDARD_ROCK]
     off = np.array([0, 20000])
     K, q = S.make_pairs(off, seed=5, device=device)
     ex = S.exact_dcs(K, q, off, els)
     net = S.Network(seed=4, device=device)
     stats = {"mean": np.zeros(4), "std": np.ones(4), "trend": np.zeros((4, S.NTREND))}
-    sur = S.Surrogate(net, els, stats, np.ones((1, S.MAP_BINS, S.MAP_BINS), np.uint8), 1e-3)
+    # every process of every cell flagged
+    sur = S.Surrogate(net, els, stats, np.full((1, S.MAP_BINS, S.MAP_BINS), 0xF, np.uint8), 1e-3)
     out, cnt = sur.evaluate(K, q, off, recheck_per_mille=0)
-    assert S.Surrogate.read_counters(cnt)[0] == 20000
+    n_exact, _ = S.Surrogate.read_counters(cnt)
+    assert abs(n_exact - _n_active(ex)) <= 40, (n_exact, _n_active(ex))
     for p in range(4):
         assert torch.equal(out[p], ex[p].float())
+    # no map, every query in the recheck sample
     sur.flag_map = None
     out, cnt = sur.evaluate(K, q, off, recheck_per_mille=1000)
     n_exact, recheck = S.Surrogate.read_counters(cnt)
-    assert n_exact == 20000 and recheck > 0  # an untrained network is far off
+    assert abs(n_exact - _n_active(ex)) <= 40 and recheck > 0  # an untrained network is far off
     for p in range(4):
         assert torch.equal(out[p], ex[p].float())
     # no routing: the zero pattern follows the exact kinematic exits; it can
     # differ only where an exact result clamps to zero inside the kinematic
     # range (every node's screening term clamped), which is rare
-    out, cnt = sur.evaluate(K, q, off, recheck_per_mille=0)
+    net_out, cnt = sur.evaluate(K, q, off, recheck_per_mille=0)
     assert S.Surrogate.read_counters(cnt)[0] == 0
     for p in range(4):
-        mismatch = ((out[p] == 0) != (ex[p] == 0)).double().mean().item()
+        mismatch = ((net_out[p] == 0) != (ex[p] == 0)).double().mean().item()
         assert mismatch < 2e-3, (p, mismatch)
+    # only photonuclear flagged: it is exact, the other three are the network's
+    sur.flag_map = torch.full((1, S.MAP_BINS, S.MAP_BINS), 1 << 2, dtype=torch.uint8, device=device)
+    out, cnt = sur.evaluate(K, q, off, recheck_per_mille=0)
+    full = S.Surrogate.read_counters_full(cnt)
+    assert full["exact_pair"] == 0 and abs(full["exact_photo"] - int((ex[2] > 0).sum().item())) <= 20, full
+    assert torch.equal(out[2], ex[2].float())
+    for p in (0, 1, 3):
+        assert torch.equal(out[p], net_out[p]), p
     # out-of-domain queries go to the exact path
+    sur.flag_map = None
     K2 = K.clone()
     K2[:100] = 5e6
+    ex2 = S.exact_dcs(K2, q, off, els)
     out, cnt = sur.evaluate(K2, q, off, recheck_per_mille=0)
-    assert S.Surrogate.read_counters(cnt)[0] == 100
+    assert abs(S.Surrogate.read_counters(cnt)[0] - _n_active(ex2, slice(0, 100))) <= 2
+    for p in range(4):
+        assert torch.equal(out[p][:100], ex2[p][:100].float()), p
     net.close()
 
 
@@ -183,17 +206,23 @@ def test_trained_network_gain(device):
     # the stated tolerance: every query within it (here by exact routing)
     out, cnt = sur.evaluate(K, q, off, recheck_per_mille=10)
     n_exact, recheck = S.Surrogate.read_counters(cnt)
+    print("stated tolerance", sur.tol, "served cells", sur.served_fraction(), "exact share of open pair+photo",
+          n_exact / _n_active(ex), "recheck max", recheck, flush=True)
+    for p in range(4):
+        nz = ex[p] > 0
+        e = (out[p].double()[nz] / ex[p][nz] - 1).abs()
+        print(f"process {p}: max {e.max().item():.3e}, above tol {(e > sur.tol).sum().item()} of {nz.sum().item()}",
+              flush=True)
     for r in S.max_rel_errors(out, ex):
         assert r["max_rel"] <= sur.tol, r
-    print("stated tolerance", sur.tol, "exact fraction", n_exact / K.numel(), flush=True)
     # 3e-2: the network serves most cells
     sur.set_tolerance(3e-2)
-    served_cells = 1.0 - float(sur.flag_map_host.mean())
+    served_cells = float(sur.served_fraction().min())
     out, cnt = sur.evaluate(K, q, off, recheck_per_mille=10)
     n_exact, recheck = S.Surrogate.read_counters(cnt)
     print("tolerance 3e-2: served cells", served_cells, "exact fraction", n_exact / K.numel(), "recheck max",
           recheck, flush=True)
-    assert served_cells > 0.5 and n_exact < 0.6 * K.numel()
+    assert served_cells > 0.5 and n_exact < 0.6 * _n_active(ex)
     for p in range(4):
         nz = ex[p] > 0
         e = (out[p].double()[nz] / ex[p][nz] - 1).abs()
