@@ -35,7 +35,7 @@ struct DcsConsts {
     // pair production
     double p_Z13, p_cut;     // Z^(1/3), mass * (1 - 0.75 sqrte Z^(1/3))
     double p_r, p_hr2;       // mass / me, 0.5 * r * r
-    double p_AZ13, p_cL, p_cLe, p_rAZ13, p_15Z13, p_r2, p_inv_r2;
+    double p_AZ13, p_cL, p_cLe, p_rAZ13, p_15Z13, p_r2, p_inv_r2, p_AZ13_sq;
     double p_g1, p_g2, p_dcs_c;  // zeta gammas, 1.794664E-34 * Z
     // photonuclear
     double ph_logA, ph_a_low, ph_halfA, ph_2m2, ph_m2, ph_thr_pi, ph_L0, ph_inv_L0;
@@ -88,6 +88,7 @@ inline DcsConsts make_consts(double A, double I, int32_t Z, double mass) {
         c.p_15Z13 = 1.5 * c.p_Z13;
         c.p_r2 = c.p_r * c.p_r;
         c.p_inv_r2 = 1. / c.p_r2;
+        c.p_AZ13_sq = c.p_AZ13 * c.p_AZ13;
         c.p_g1 = (Z == 1) ? 4.4E-05 : 1.95E-05;
         c.p_g2 = (Z == 1) ? 4.8E-05 : 5.30E-05;
         c.p_dcs_c = 1.794664E-34 * double(Z);

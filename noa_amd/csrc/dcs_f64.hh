@@ -2,18 +2,25 @@
 //
 // FMA contraction is off in every function body and the closed forms keep
 // the reference's evaluation order. The node loops of the two integrals are
-// rewritten algebraically to spend fewer divisions and logarithms (the same
-// mathematics with a few ulps of different rounding, far inside the 1e-10
-// parity bound):
-//   pair production  the two screening logarithms of L_e become one log of
-//                    A Z^-1/3 q rho21 sqrt(x_e / ((1 + c_Le x_e)
-//                    (q rho21 + c_L x_e)^2)); c_L / rho21 is folded into the
-//                    log arguments; (rho21 - beta) / (1 + xi) is shared by
-//                    B_e and B_mu; / r^2 is a product with 1 / r^2;
-//   photonuclear     per-pair reciprocals of M q, E^2 and q; W^2 - M^2 =
-//                    2 M q - Q^2; one reciprocal for x_P and x_R, one for the
-//                    two ALLM logistic factors, one for the three terms of
-//                    Whitlow's R, one 1 / Q^2; log10 as log / ln 10.
+// rewritten algebraically to spend fewer divisions, square roots and
+// logarithms (the same mathematics with a few ulps of different rounding,
+// far inside the 1e-10 parity bound):
+//   pair production  the two screening logarithms of L_e become one,
+//                    0.5 log(A^2 Z^-2/3 q^2 rho21^2 x_e / ((1 + c_Le x_e)
+//                    (q rho21 + c_L x_e)^2)), with no square root; c_L / rho21
+//                    is folded into the log arguments; (rho21 - beta) / (1 +
+//                    xi) is shared by B_e and B_mu; / r^2 is a product with
+//                    1 / r^2; each node takes 1 / xi as the reference does and
+//                    two reciprocals of products (the two Y denominators with
+//                    1 + xi; the two screening denominators) where the
+//                    reference divides five more times;
+//   photonuclear     W^2 - M^2 + Q^2 = 2 M q, so the ALLM denominators of x_P
+//                    and x_R are per-pair constants; one per-pair reciprocal
+//                    gives 1 / q, 1 / E^2 and those two; in a node, Whitlow's
+//                    R and theta are carried as fractions, so that 1 / (1 + R)
+//                    is D_R D_theta / S, and one reciprocal of Q^2 (Q^2 + m0^2)
+//                    g_P g_A S replaces the reference's eight divisions; log10
+//                    as log / ln 10.
 // Otherwise the device log/exp (the table-driven ones of fastmath.hh, ~0.5
 // ulp) against glibc is the only difference from the reference's CPU path.
 // Launch invariants come precomputed in DcsConsts.
@@ -130,32 +137,44 @@ struct F64 {
             const double rho2 = rho * rho;
             const double rho21 = eps * (2. - eps);
             const double xi = xi_factor * rho21;
+            const double xi1 = 1. + xi;
+            // correctly rounded, as in the reference: near xi = 1e3, B_e's
+            // log(1 + 1/xi) - ... cancels ~3000-fold, so one ulp of 1/xi that
+            // changes the rounding of 1 + 1/xi moves B_e by ~1e-10
             const double xi_i = 1. / xi;
-            const double dr = (rho21 - beta) / (1. + xi);  // shared by B_e and B_mu
+            // Y_e = N_e / D_e, Y_mu = N_mu / D_mu and 1 / (1 + xi) over one
+            // reciprocal (D_e, D_mu > 1)
+            const double De = 2. * (1. + 3. * beta) * fm::log(3. + xi_i, t) - rho2 - 2. * beta * (2. - rho2);
+            const double Dm = (1. + rho2) * (1.5 + 2. * beta) * fm::log(3. + xi, t) + 1. - 1.5 * rho2;
+            const double DD = De * Dm;
+            const double r_y = 1. / (DD * xi1);
+            const double dr = (rho21 - beta) * (DD * r_y);  // (rho21 - beta) / (1 + xi), shared by B_e and B_mu
             double Be;
             if (xi >= 1E+03)
                 Be = 0.5 * xi_i * ((3. - rho2) + 2. * beta * (1. + rho2));
             else
                 Be = ((2. + rho2) * (1. + beta) + xi * (3. + rho2)) * fm::log(1. + xi_i, t) + dr - 3. - rho2;
-            const double Ye = (5. - rho2 + 4. * beta * (1. + rho2)) /
-                              (2. * (1. + 3. * beta) * fm::log(3. + xi_i, t) - rho2 - 2. * beta * (2. - rho2));
-            const double xe = (1. + xi) * (1. + Ye);
-            // q + (c_L / rho21) x = (q rho21 + c_L x) / rho21
-            const double qr = q * rho21;
-            const double ge = qr + c.p_cL * xe;
-            const double Le = fm::log(c.p_AZ13 * qr * sqrt(xe / ((1. + c.p_cLe * xe) * (ge * ge))), t);
-            double Phi_e = Be * Le;
-            if (Phi_e < 0.) Phi_e = 0.;
             double Bmu;
             if (xi <= 1E-03)
                 Bmu = 0.5 * xi * (5. - rho2 + beta * (3. + rho2));
             else
                 Bmu = ((1. + rho2) * (1. + 1.5 * beta) - xi_i * (1. + 2. * beta) * rho21) * fm::log(1. + xi, t) +
                       xi * dr + (1. + 2. * beta) * rho21;
-            const double Ymu = (4. + rho2 + 3. * beta * (1. + rho2)) /
-                               ((1. + rho2) * (1.5 + 2. * beta) * fm::log(3. + xi, t) + 1. - 1.5 * rho2);
-            const double xmu = (1. + xi) * (1. + Ymu);
-            const double Lmu = fm::log(c.p_rAZ13 * qr / (c.p_15Z13 * (qr + c.p_cL * xmu)), t);
+            const double Ye = (5. - rho2 + 4. * beta * (1. + rho2)) * ((Dm * xi1) * r_y);
+            const double Ymu = (4. + rho2 + 3. * beta * (1. + rho2)) * ((De * xi1) * r_y);
+            const double xe = xi1 * (1. + Ye);
+            const double xmu = xi1 * (1. + Ymu);
+            // q + (c_L / rho21) x = (q rho21 + c_L x) / rho21; the two screening
+            // denominators over one reciprocal
+            const double qr = q * rho21;
+            const double ge = qr + c.p_cL * xe;
+            const double Dle = (1. + c.p_cLe * xe) * (ge * ge);
+            const double Dlm = c.p_15Z13 * (qr + c.p_cL * xmu);
+            const double r_l = 1. / (Dle * Dlm);
+            const double Le = 0.5 * fm::log(c.p_AZ13_sq * (qr * qr) * xe * (Dlm * r_l), t);
+            double Phi_e = Be * Le;
+            if (Phi_e < 0.) Phi_e = 0.;
+            const double Lmu = fm::log(c.p_rAZ13 * qr * (Dle * r_l), t);
             double Phi_mu = Bmu * Lmu;
             if (Phi_mu < 0.) Phi_mu = 0.;
             const double f = -(Phi_e + Phi_mu * c.p_inv_r2) * (1. - rho) * tmin;
@@ -203,12 +222,19 @@ struct F64 {
         const double dpQ2 = pQ2max - pQ2min;
         const double pQ2c = 0.5 * (pQ2max + pQ2min);
         const double hd = 0.5 * dpQ2;
-        const double Mq = M * q;
         const double EE = E * E;
-        const double twoMq = 2. * Mq;
-        const double inv_2Mq = 0.5 / Mq;
-        const double inv_EE = 1. / EE;
-        const double inv_q = 1. / q;
+        const double twoMq = 2. * M * q;
+        // x_P = (Q^2 + m_P^2) / (Q^2 + m_P^2 + W^2 - M^2) with W^2 - M^2 = 2 M q - Q^2
+        // (dcs.hh:290-293): per-pair denominators, with 1 / q and 1 / E^2 over
+        // one reciprocal
+        const double dP = twoMq + 49.457, dR = twoMq + 0.15052;
+        const double qE = q * EE, PR = dP * dR;
+        const double r_pair = 1. / (qE * PR);
+        const double inv_q = EE * (PR * r_pair);
+        const double inv_EE = q * (PR * r_pair);
+        const double inv_dP = dR * (qE * r_pair);
+        const double inv_dR = dP * (qE * r_pair);
+        const double inv_2Mq = inv_q * (0.5 / M);
         const double inv_l2 = 1. / ALLM_LAMBDA2;
 
         double ds = 0.;
@@ -217,27 +243,43 @@ struct F64 {
             const double Q2 = fm::exp(pQ2c + hd * GL9_X[j], t);
             // dcs_photonuclear_d2 (dcs.hh:335-355)
             const double x = Q2 * inv_2Mq;
-            // dcs_photonuclear_f2_allm (dcs.hh:261-307); W^2 - M^2 = 2 M q - Q^2
-            const double D = twoMq - Q2;
+            // dcs_photonuclear_f2_allm (dcs.hh:261-307)
             const double tl = fm::log(fm::log((Q2 + ALLM_Q02) * inv_l2, t) * c.ph_inv_L0, t);
-            const double nP = Q2 + 49.457, nR = Q2 + 0.15052;
-            const double dP = nP + D, dR = nR + D;
-            const double rPR = 1.0 / (dP * dR);
-            const double xP = nP * dR * rPR;
-            const double xR = nR * dP * rPR;
+            const double xP = (Q2 + 49.457) * inv_dP;
+            const double xR = (Q2 + 0.15052) * inv_dR;
             const double lnt = fm::log(tl, t);
             const double gP = 1.0 + fm::exp(2.1979 * lnt, t), gA = 1.0 + fm::exp(1.1709 * lnt, t);
-            const double rG = 1.0 / (gP * gA);
-            const double cP = 0.28067 + (0.28067 - 0.22291) * (gA * rG - 1.0);
-            const double aP = -0.0808 + (-0.0808 - -0.44812) * (gP * rG - 1.0);
             const double bP = 0.36292 + 1.8917 * fm::exp(1.8439 * lnt, t);
             const double cR = 0.80107 + 0.97307 * fm::exp(3.4942 * lnt, t);
             const double aR = 0.58400 + 0.37888 * fm::exp(2.6063 * lnt, t);
             const double bR = 0.01147 + 3.7582 * fm::exp(0.49338 * lnt, t);
+            // dcs_photonuclear_r_whitlow (dcs.hh:322-332) as fractions:
+            // theta = (D_th + 12 c q2) / D_th, R = N_R / D_R, so
+            // 1 / (1 + R) = D_R D_th / S
+            double q2 = Q2;
+            if (Q2 < 0.3) q2 = 0.3;
+            const double lg = fm::log(q2 * 25.0, t);  // q2 / 0.04
+            const double den = 0.09 + q2 * q2;
+            const double Dth = (1.0 + q2) * (0.015625 + x * x);
+            const double DR = lg * q2 * den;
+            const double S = Dth * (DR + lg * (0.5747 * den - 0.3534 * q2)) +
+                             0.635 * q2 * den * (Dth + (12.0 * 0.015625) * q2);
+            // 1 / Q^2, 1 / (Q^2 + m0^2), 1 / (g_P g_A) and 1 / S over one reciprocal
+            const double Qm = Q2 + 0.31985;
+            const double gPA = gP * gA;
+            const double QQm = Q2 * Qm, gS = gPA * S;
+            const double r = 1.0 / (QQm * gS);
+            const double rq = gS * r, rg = QQm * r;
+            const double inv_Q2 = Qm * rq;
+            const double inv_Qm = Q2 * rq;
+            const double rG = S * rg;
+            const double inv_S = gPA * rg;
+            const double cP = 0.28067 + (0.28067 - 0.22291) * (gA * rG - 1.0);
+            const double aP = -0.0808 + (-0.0808 - -0.44812) * (gP * rG - 1.0);
             const double l1x = fm::log(1. - x, t);
             const double F2P = cP * fm::exp(aP * fm::log(xP, t) + bP * l1x, t);
             const double F2R = cR * fm::exp(aR * fm::log(xR, t) + bR * l1x, t);
-            const double F2p = Q2 / (Q2 + 0.31985) * (F2P + F2R);
+            const double F2p = Q2 * inv_Qm * (F2P + F2R);
             // dcs_photonuclear_f2a_drss (dcs.hh:310-319)
             double a = 1.0;
             if (x < 0.0014)
@@ -245,17 +287,9 @@ struct F64 {
             else if (x < 0.04)
                 a = fm::exp((0.069 * (fm::log(x, t) * INV_LN10) + 0.097) * c.ph_logA, t);
             const double F2A = c.ph_halfA * a * (2.0 + x * (-1.85 + x * (2.45 + x * (-2.35 + x)))) * F2p;
-            // dcs_photonuclear_r_whitlow (dcs.hh:322-332), over one reciprocal
-            double q2 = Q2;
-            if (Q2 < 0.3) q2 = 0.3;
-            const double theta = 1. + (12.0 * 0.015625) * q2 / ((1.0 + q2) * (0.015625 + x * x));
-            const double lg = fm::log(q2 * 25.0, t);  // q2 / 0.04
-            const double den = 0.09 + q2 * q2;
-            const double rR = 1.0 / (lg * q2 * den);
-            const double R = (0.635 * theta) * (q2 * den) * rR + 0.5747 * (lg * den) * rR - 0.3534 * (lg * q2) * rR;
-            const double inv_Q2 = 1.0 / Q2;
+            const double inv_1R = DR * Dth * inv_S;
             const double dds =
-                (1. - y + 0.5 * (1. - c.ph_2m2 * inv_Q2) * (y * y + Q2 * inv_EE) / (1. + R)) * (inv_Q2 * inv_Q2) -
+                (1. - y + 0.5 * (1. - c.ph_2m2 * inv_Q2) * (y * y + Q2 * inv_EE) * inv_1R) * (inv_Q2 * inv_Q2) -
                 0.25 * inv_EE * inv_Q2;
             const double d2 = 2.603096E-35 * F2A * dds * inv_q;
             ds += d2 * Q2 * 1. * GL9_W[j];

@@ -2,7 +2,7 @@
 //
 // ocml's FP64 log is ~98 VALU instructions and exp ~42 on gfx950, and the DCS
 // path makes ~52 log and ~24 exp calls per (K, q) pair, so libm dominates the
-// VALU issue budget. These versions take ~25 (log) and ~20 (exp) VALU
+// VALU issue budget. These versions take ~22 (log) and ~20 (exp) VALU
 // instructions plus LDS table reads, within 0.75 ulp (log) and 0.6 ulp (exp).
 // Special arguments are handled in place, without an inlined libm fallback at
 // every call site: log of 0, negatives, inf and NaN by their IEEE values, of
@@ -10,12 +10,14 @@
 // +-708 runs the same path, with ldexp producing the overflow, subnormal and
 // zero results.
 //
-// log x = k ln2 + log(1/invc_i) + log1p(r),  r = z invc_i - 1, |r| < 2^-7,
-//   with x = 2^k z, z in [0.6875, 1.375) and i from the top 7 mantissa bits
+// log x = k ln2 + log(1/invc_i) + log1p(r),  r = z invc_i - 1 in [-2^-9, 2^-8],
+//   with x = 2^k z, z in [0.6875, 1.375) and i from the top 8 mantissa bits
 //   (invc = 1 on the two subintervals that touch 1, so r = z - 1 exactly and
 //   log x keeps full relative accuracy as x -> 1); k LN2_HI + logc_hi is
 //   exact (both multiples of 2^-43), the small parts are summed before they
-//   are added to it, and log1p(r) - r is a degree-9 Taylor polynomial.
+//   are added to it, and log1p(r) - r = r^2 P(r) with P the degree-4
+//   Chebyshev economization of the series over that r range (truncation
+//   error < 0.07 * 2^-53 relative; coefficients from gen_fastmath_tables.py).
 // exp x = 2^m 2^(j/128) exp(r), k = rint(128 x / ln2) = 128 m + j,
 //   r = x - k ln2/128 (two FMAs), exp(r) - 1 a degree-5 Taylor polynomial.
 //
@@ -122,20 +124,17 @@ NOA_HD inline double log(double x, const Tab& t) {
         kadj = -54;
     }
     const uint64_t tmp = ix - LOG_OFF;
-    const int i = int((tmp >> 45) & uint64_t(LOG_N - 1));
+    const int i = int((tmp >> LOG_SHIFT) & uint64_t(LOG_N - 1));
     const int k = int(int64_t(tmp) >> 52) + kadj;
     const double z = from_bits(ix - (tmp & 0xfff0000000000000ull));
     const double kd = double(k);
     const double r = std::fma(z, t.invc[i], -1.0);
     const double w = fma_kb(kd, LN2_HI, t.lhi[i]);  // exact
-    double q = fma_kc(r, 1.0 / 9.0, -1.0 / 8.0);
-    q = fma_kc(q, r, 1.0 / 7.0);
-    q = fma_kc(q, r, -1.0 / 6.0);
-    q = fma_kc(q, r, 1.0 / 5.0);
-    q = fma_kc(q, r, -1.0 / 4.0);
-    q = fma_kc(q, r, 1.0 / 3.0);
-    q = std::fma(q, r, -0.5);
-    // w exact and |r| < 2^-7: summing the small parts first leaves two
+    double q = fma_kc(r, LOG_P4, LOG_P3);
+    q = fma_kc(q, r, LOG_P2);
+    q = fma_kc(q, r, LOG_P1);
+    q = fma_kc(q, r, LOG_P0);
+    // w exact and |r| <= 2^-8: summing the small parts first leaves two
     // roundings, <= 0.75 ulp in all (a TwoSum of w + r would buy 0.15 ulp
     // for five more additions)
     const double lo = fma_kb(kd, LN2_LO, t.llo[i]);

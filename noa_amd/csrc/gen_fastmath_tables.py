@@ -5,13 +5,21 @@ log and exp in fastmath.hh (Tang's method).
 
 All values are computed with 60-digit decimal arithmetic and rounded once.
 
-log: z in [0.6875, 1.375) is cut into 128 subintervals by the top 7 mantissa
+log: z in [0.6875, 1.375) is cut into 256 subintervals by the top 8 mantissa
 bits of (bits(x) - bits(0.6875)); for subinterval i with centre c_i (c = 1
 for the two subintervals that touch 1),
   invc[i]  = 1 / c_i rounded to double,
   logc_hi  = -log(invc[i]) rounded to a multiple of 2^-43 (so that
              k * LN2_HI + logc_hi is exact for |k| < 2^11),
   logc_lo  = the remainder, rounded to double.
+  Then r = z invc - 1 lies in [-2^-9, 2^-8] (the widest ranges are those of
+  the two subintervals at 1, where r = z - 1 exactly).
+log1p(r) = r + r^2 P(r): P is the degree-4 Chebyshev economization of the
+  series (log1p(r) - r) / r^2 = sum_n (-1)^(n+1) r^n / (n + 2) over that r
+  range (exact rational arithmetic: the series to degree 40 in the shifted
+  and scaled variable, its Chebyshev expansion cut after T_4, converted back
+  to powers of r), coefficients rounded once; the truncation error is below
+  0.07 * 2^-53 relative to log1p(r) (checked here against 60-digit ln).
 exp: T[j] = 2^(j/128), j < 128, as a double-double (hi, lo).
 """
 from __future__ import annotations
@@ -19,10 +27,15 @@ from __future__ import annotations
 import os
 from decimal import Decimal, getcontext
 
+from fractions import Fraction
+from math import comb
+
 getcontext().prec = 60
 HERE = os.path.dirname(os.path.abspath(__file__))
-N_LOG = 128
+N_LOG = 256
+LOG_SHIFT = 44  # 52 - log2(N_LOG)
 N_EXP = 128
+LOG_DEG = 4
 OFF = 0x3fe6000000000000  # bits of 0.6875
 
 
@@ -48,8 +61,8 @@ def log_tables():
     invc, lhi, llo = [], [], []
     q43 = Decimal(2) ** -43
     for i in range(N_LOG):
-        lo_b = OFF + (i << 45)
-        hi_b = OFF + ((i + 1) << 45)
+        lo_b = OFF + (i << LOG_SHIFT)
+        hi_b = OFF + ((i + 1) << LOG_SHIFT)
         a = Decimal(bits_to_float(lo_b))
         b = Decimal(bits_to_float(hi_b))
         c = (a + b) / 2
@@ -64,6 +77,67 @@ def log_tables():
         lhi.append(d2f(h))
         llo.append(d2f(lc - Decimal(d2f(h))))
     return invc, lhi, llo
+
+
+def _to_chebyshev(p):
+    """Power coefficients in s -> Chebyshev coefficients (Horner in the T basis)."""
+    n = len(p) - 1
+    c = [Fraction(0)] * (n + 2)
+    for k in range(n, -1, -1):
+        nc = [Fraction(0)] * (n + 2)
+        for j, v in enumerate(c):
+            if v == 0:
+                continue
+            if j == 0:
+                nc[1] += v
+            else:  # s T_j = (T_(j+1) + T_(j-1)) / 2
+                nc[j + 1] += v / 2
+                nc[j - 1] += v / 2
+        nc[0] += p[k]
+        c = nc
+    return c
+
+
+def _chebyshev_to_power(c):
+    T = [[Fraction(1)], [Fraction(0), Fraction(1)]]
+    for j in range(2, len(c)):
+        a = [Fraction(0)] + [2 * v for v in T[j - 1]]
+        b = T[j - 2] + [Fraction(0)] * (len(a) - len(T[j - 2]))
+        T.append([x - y for x, y in zip(a, b)])
+    p = [Fraction(0)] * len(c)
+    for j in range(len(c)):
+        for k, v in enumerate(T[j]):
+            p[k] += c[j] * v
+    return p
+
+
+def log_poly(lo: Fraction, hi: Fraction, deg: int = LOG_DEG, terms: int = 40):
+    """Coefficients (constant first) of P(r) ~ (log1p(r) - r) / r^2 on [lo, hi]."""
+    m, h = (lo + hi) / 2, (hi - lo) / 2
+    ps = [Fraction(0)] * (terms + 1)  # the series in s, r = m + h s
+    for n in range(terms + 1):
+        cn = Fraction((-1) ** (n + 1), n + 2)
+        for k in range(n + 1):
+            ps[k] += cn * comb(n, k) * m ** (n - k) * h ** k
+    p_s = _chebyshev_to_power(_to_chebyshev(ps)[:deg + 1])
+    pr = [Fraction(0)] * (deg + 1)  # back to r: s = (r - m) / h
+    for k, v in enumerate(p_s):
+        for j in range(k + 1):
+            pr[j] += v * comb(k, j) * (1 / h) ** k * (-m) ** (k - j)
+    coef = [float(v) for v in pr]
+    worst = Decimal(0)
+    for i in range(2001):
+        r = lo + (hi - lo) * Fraction(i, 2000)
+        if r == 0:
+            continue
+        q = Fraction(0)
+        for v in reversed(coef):
+            q = q * r + Fraction(v)
+        approx = r + r * r * q
+        exact = (1 + Decimal(r.numerator) / Decimal(r.denominator)).ln()
+        worst = max(worst, abs((Decimal(approx.numerator) / Decimal(approx.denominator) - exact) / exact))
+    assert worst < Decimal(2) ** -56, worst
+    return coef
 
 
 def exp_tables():
@@ -110,10 +184,15 @@ def main() -> None:
         "namespace fm {",
         "",
         f"constexpr int LOG_N = {N_LOG};",
+        f"constexpr int LOG_SHIFT = {LOG_SHIFT};",
         f"constexpr int EXP_N = {N_EXP};",
     ]
     for k, v in c.items():
         out.append(f"constexpr double {k} = {float.hex(v)};")
+    poly = log_poly(-Fraction(1, 2 * N_LOG), Fraction(1, N_LOG))
+    out.append("// log1p(r) = r + r^2 (LOG_P0 + r (LOG_P1 + ...)), r in [-2^-9, 2^-8]")
+    for k, v in enumerate(poly):
+        out.append(f"constexpr double LOG_P{k} = {float.hex(v)};")
     for name, vals in (("LOG_INVC", invc), ("LOG_LOGC_HI", lhi), ("LOG_LOGC_LO", llo),
                        ("EXP_T_HI", th), ("EXP_T_LO", tl)):
         out += ["", f"NOA_FM_TABLE double {name}[{len(vals)}] = {{", fmt(vals), "};"]

@@ -394,8 +394,11 @@ __global__ __launch_bounds__(IWG) void surrogate_dcs_kernel(
 
 // Exact FP64 pair production (entries [0, counters[1])) and photonuclear
 // (entries [cap, cap + counters[3])) of the fallback lists; every entry is
-// kinematically active. For recheck entries the network's value, already in
-// the output, is compared first.
+// kinematically active. A wave takes 64 consecutive entries of one list, so
+// it runs one integral; its lanes are grouped by element (the lists follow
+// query order, so a wave mostly holds one), and each group reads its
+// constants through scalar loads, as the map kernels do. For recheck
+// entries the network's value, already in the output, is compared first.
 __global__ __launch_bounds__(256) void fallback_kernel(const double* __restrict__ K, const double* __restrict__ q,
                                                        const SurElem* __restrict__ els,
                                                        const int64_t* __restrict__ fb_idx,
@@ -406,19 +409,39 @@ __global__ __launch_bounds__(256) void fallback_kernel(const double* __restrict_
     __syncthreads();
     const fm::Tab tab = fm::tab_from_block(fm_block);
     const int64_t n_pair = counters[1], n_photo = counters[3];
+    const int64_t w_pair = (n_pair + 63) / 64, w_all = w_pair + (n_photo + 63) / 64;
+    const int lane = threadIdx.x % 64;
+    const int64_t waves = int64_t(gridDim.x) * (blockDim.x / 64);
     float worst = 0.f;
-    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n_pair + n_photo;
-         i += int64_t(gridDim.x) * blockDim.x) {
-        const bool pair = i < n_pair;
-        const int64_t slot = pair ? i : cap + (i - n_pair);
-        const int64_t g = fb_idx[slot];
-        const uint32_t tag = fb_el[slot];
-        const DcsConsts& c = els[tag & ~FB_RECHECK].c;
-        const double k = K[g], qq = q[g];
-        const double v = pair ? F64::pair(k, qq, c, tab) : F64::photo(k, qq, c, tab);
+    using ConstPtr = const __attribute__((address_space(4))) SurElem*;
+    for (int64_t w = int64_t(blockIdx.x) * (blockDim.x / 64) + threadIdx.x / 64; w < w_all; w += waves) {
+        const bool pair = w < w_pair;  // wave-uniform
+        const int64_t i = (pair ? w : w - w_pair) * 64 + lane;
+        bool todo = i < (pair ? n_pair : n_photo);
+        const int64_t slot = pair ? i : cap + i;
+        int64_t g = 0;
+        uint32_t tag = 0;
+        double k = 1., qq = 0.5;
+        if (todo) {
+            g = fb_idx[slot];
+            tag = fb_el[slot];
+            k = K[g];
+            qq = q[g];
+        }
+        const uint32_t el = tag & ~FB_RECHECK;
         float* dst = pair ? o.p : o.ph;
-        if ((tag & FB_RECHECK) && v > 0.) worst = fmaxf(worst, float(fabs(double(dst[g]) / v - 1.)));
-        dst[g] = float(v);
+        for (;;) {
+            const unsigned long long m = __ballot(todo);
+            if (!m) break;
+            const uint32_t e = uint32_t(__builtin_amdgcn_readlane(int(el), __ffsll((long long)m) - 1));
+            const DcsConsts& c = ((const SurElem*)((ConstPtr)(els + e)))->c;
+            if (todo && el == e) {
+                const double v = pair ? F64::pair(k, qq, c, tab) : F64::photo(k, qq, c, tab);
+                if ((tag & FB_RECHECK) && v > 0.) worst = fmaxf(worst, float(fabs(double(dst[g]) / v - 1.)));
+                dst[g] = float(v);
+                todo = false;
+            }
+        }
     }
     if (worst > 0.f) atomicMax(&counters[2], __float_as_uint(worst));
 }

@@ -115,7 +115,18 @@ def test_fresh_inputs_against_oracle(device):
             assert _rel(t.cpu().numpy(), r) < RTOL_CLOSED
         Gk = torch.zeros_like(fCM)
         dcs.coulomb_transport(Gk, scr, fs, torch.from_numpy(mu).to(device))
-        assert _rel(Gk.cpu().numpy(), oracle.coulomb_transport(ref[1], ref[2], mu)) < RTOL_CLOSED
+        Gk = Gk.cpu().numpy()
+        Gr = oracle.coulomb_transport(ref[1], ref[2], mu)
+        assert _rel(Gk[:, 0], Gr[:, 0]) < RTOL_CLOSED
+        # G[1] on the small-mu branch (dcs.hh:630-637) is 2k (L - r - ...)
+        # with L = log(1 + x), r = x / (1 + x), x = mu / s0: L - r ~ x^2 / 2,
+        # so one ulp of difference in the log (table log against glibc) moves
+        # G[1] by up to ~2 eps / x relative; the bound follows that conditioning
+        x = mu / ref[1][:, 0]
+        small = mu < 1e-8 * np.minimum(ref[1][:, 1], ref[1][:, 2])
+        bound = RTOL_CLOSED + np.where(small, 4 * np.finfo(np.float64).eps / x, 0.0)
+        err = np.abs(Gk[:, 1] - Gr[:, 1]) / np.maximum(np.abs(Gr[:, 1]), 1e-300)
+        assert (err <= bound).all(), (err / bound).max()
         G = torch.zeros_like(fCM)
         dcs.coulomb_transport(G, scr, fs, torch.tensor([1.0], dtype=torch.float64, device=device))
         per_gpu.append((G, fCM, scr, il, fs))
