@@ -21,6 +21,14 @@
 //                    is D_R D_theta / S, and one reciprocal of Q^2 (Q^2 + m0^2)
 //                    g_P g_A S replaces the reference's eight divisions; log10
 //                    as log / ln 10.
+// Inside the node loops, log_pos / exp_fin (no special-argument paths) take
+// the arguments that are positive normal / finite by construction for total
+// energies up to 1e30 GeV: 1 + 1/xi, 3 + xi, the screening arguments, Q^2's
+// exponent, (Q^2 + Q_0^2) / Lambda^2 >= 8, 1 - x >= m_pi / q, x_P, x_R and
+// the ALLM exponents. NaN inputs still give NaN results through the
+// arithmetic around them. Above 1e30 GeV, and for infinite or NaN K, the
+// integrals run with the full log / exp (the same code, templated), so the
+// reference's overflow results are kept.
 // Otherwise the device log/exp (the table-driven ones of fastmath.hh, ~0.5
 // ulp) against glibc is the only difference from the reference's CPU path.
 // Launch invariants come precomputed in DcsConsts.
@@ -115,8 +123,30 @@ struct F64 {
         return !((argmin >= 1.) || (argmin <= 0.));
     }
 
+    // Node-loop log / exp: the full versions on the safe path, the ones
+    // without special-argument handling otherwise.
+    template <bool Safe>
+    static __device__ __forceinline__ double nlog(double x, const fm::Tab& t) {
+        return Safe ? fm::log(x, t) : fm::log_pos(x, t);
+    }
+    template <bool Safe>
+    static __device__ __forceinline__ double nexp(double x, const fm::Tab& t) {
+        return Safe ? fm::exp(x, t) : fm::exp_fin(x, t);
+    }
+
+    // Total energy up to which every node-loop argument of the two integrals
+    // stays positive normal / finite (bounds: beta >= 1e-66, the screening
+    // denominators <= 1e140, 1 - x >= 1e-31).
+    static constexpr double FAST_E_MAX = 1e30;
+
     // dcs.hh:144-258 for a pair that passed pair_active.
     static __device__ double pair(double K, double q, const DcsConsts& c, const fm::Tab& t) {
+        if (K + c.mass <= FAST_E_MAX) return pair_impl<false>(K, q, c, t);
+        return pair_impl<true>(K, q, c, t);  // also infinite and NaN K
+    }
+
+    template <bool Safe>
+    static __device__ double pair_impl(double K, double q, const DcsConsts& c, const fm::Tab& t) {
 #pragma clang fp contract(off)
         const double me = ELECTRON_MASS;
         const double nu = q / (K + c.mass);
@@ -132,7 +162,7 @@ struct F64 {
 #pragma unroll 1
         for (int j = 0; j < 8; j++) {
             const double tj = GL8_X[j];
-            const double eps = fm::exp(tj * tmin, t);
+            const double eps = nexp<Safe>(tj * tmin, t);
             const double rho = 1. - eps;
             const double rho2 = rho * rho;
             const double rho21 = eps * (2. - eps);
@@ -142,39 +172,51 @@ struct F64 {
             // log(1 + 1/xi) - ... cancels ~3000-fold, so one ulp of 1/xi that
             // changes the rounding of 1 + 1/xi moves B_e by ~1e-10
             const double xi_i = 1. / xi;
-            // Y_e = N_e / D_e, Y_mu = N_mu / D_mu and 1 / (1 + xi) over one
-            // reciprocal (D_e, D_mu > 1)
-            const double De = 2. * (1. + 3. * beta) * fm::log(3. + xi_i, t) - rho2 - 2. * beta * (2. - rho2);
-            const double Dm = (1. + rho2) * (1.5 + 2. * beta) * fm::log(3. + xi, t) + 1. - 1.5 * rho2;
-            const double DD = De * Dm;
-            const double r_y = 1. / (DD * xi1);
-            const double dr = (rho21 - beta) * (DD * r_y);  // (rho21 - beta) / (1 + xi), shared by B_e and B_mu
+            const double De = 2. * (1. + 3. * beta) * nlog<Safe>(3. + xi_i, t) - rho2 - 2. * beta * (2. - rho2);
+            const double Dm = (1. + rho2) * (1.5 + 2. * beta) * nlog<Safe>(3. + xi, t) + 1. - 1.5 * rho2;
+            const double Ne = 5. - rho2 + 4. * beta * (1. + rho2);
+            const double Nm = 4. + rho2 + 3. * beta * (1. + rho2);
+            double dr, Ye, Ymu;  // dr = (rho21 - beta) / (1 + xi), shared by B_e and B_mu
+            if constexpr (Safe) {
+                dr = (rho21 - beta) / xi1;
+                Ye = Ne / De;
+                Ymu = Nm / Dm;
+            } else {  // one reciprocal for the three (D_e, D_mu > 1)
+                const double DD = De * Dm;
+                const double r_y = 1. / (DD * xi1);
+                dr = (rho21 - beta) * (DD * r_y);
+                Ye = Ne * ((Dm * xi1) * r_y);
+                Ymu = Nm * ((De * xi1) * r_y);
+            }
             double Be;
             if (xi >= 1E+03)
                 Be = 0.5 * xi_i * ((3. - rho2) + 2. * beta * (1. + rho2));
             else
-                Be = ((2. + rho2) * (1. + beta) + xi * (3. + rho2)) * fm::log(1. + xi_i, t) + dr - 3. - rho2;
+                Be = ((2. + rho2) * (1. + beta) + xi * (3. + rho2)) * nlog<Safe>(1. + xi_i, t) + dr - 3. - rho2;
             double Bmu;
             if (xi <= 1E-03)
                 Bmu = 0.5 * xi * (5. - rho2 + beta * (3. + rho2));
             else
-                Bmu = ((1. + rho2) * (1. + 1.5 * beta) - xi_i * (1. + 2. * beta) * rho21) * fm::log(1. + xi, t) +
+                Bmu = ((1. + rho2) * (1. + 1.5 * beta) - xi_i * (1. + 2. * beta) * rho21) * nlog<Safe>(1. + xi, t) +
                       xi * dr + (1. + 2. * beta) * rho21;
-            const double Ye = (5. - rho2 + 4. * beta * (1. + rho2)) * ((Dm * xi1) * r_y);
-            const double Ymu = (4. + rho2 + 3. * beta * (1. + rho2)) * ((De * xi1) * r_y);
             const double xe = xi1 * (1. + Ye);
             const double xmu = xi1 * (1. + Ymu);
-            // q + (c_L / rho21) x = (q rho21 + c_L x) / rho21; the two screening
-            // denominators over one reciprocal
+            // q + (c_L / rho21) x = (q rho21 + c_L x) / rho21
             const double qr = q * rho21;
             const double ge = qr + c.p_cL * xe;
             const double Dle = (1. + c.p_cLe * xe) * (ge * ge);
             const double Dlm = c.p_15Z13 * (qr + c.p_cL * xmu);
-            const double r_l = 1. / (Dle * Dlm);
-            const double Le = 0.5 * fm::log(c.p_AZ13_sq * (qr * qr) * xe * (Dlm * r_l), t);
+            double Le, Lmu;
+            if constexpr (Safe) {
+                Le = fm::log(c.p_AZ13 * qr * sqrt(xe / Dle), t);
+                Lmu = fm::log(c.p_rAZ13 * qr / Dlm, t);
+            } else {  // one reciprocal for the two screening denominators, no square root
+                const double r_l = 1. / (Dle * Dlm);
+                Le = 0.5 * fm::log_pos(c.p_AZ13_sq * (qr * qr) * xe * (Dlm * r_l), t);
+                Lmu = fm::log_pos(c.p_rAZ13 * qr * (Dle * r_l), t);
+            }
             double Phi_e = Be * Le;
             if (Phi_e < 0.) Phi_e = 0.;
-            const double Lmu = fm::log(c.p_rAZ13 * qr * (Dle * r_l), t);
             double Phi_mu = Bmu * Lmu;
             if (Phi_mu < 0.) Phi_mu = 0.;
             const double f = -(Phi_e + Phi_mu * c.p_inv_r2) * (1. - rho) * tmin;
@@ -210,6 +252,12 @@ struct F64 {
 
     // dcs.hh:261-405 for a pair that passed photo_active.
     static __device__ double photo(double K, double q, const DcsConsts& c, const fm::Tab& t) {
+        if (K + c.mass <= FAST_E_MAX) return photo_impl<false>(K, q, c, t);
+        return photo_impl<true>(K, q, c, t);
+    }
+
+    template <bool Safe>
+    static __device__ double photo_impl(double K, double q, const DcsConsts& c, const fm::Tab& t) {
 #pragma clang fp contract(off)
         const double M = PHOTO_M;
         const double mpi = PHOTO_MPI;
@@ -225,69 +273,86 @@ struct F64 {
         const double EE = E * E;
         const double twoMq = 2. * M * q;
         // x_P = (Q^2 + m_P^2) / (Q^2 + m_P^2 + W^2 - M^2) with W^2 - M^2 = 2 M q - Q^2
-        // (dcs.hh:290-293): per-pair denominators, with 1 / q and 1 / E^2 over
-        // one reciprocal
+        // (dcs.hh:290-293): per-pair denominators
         const double dP = twoMq + 49.457, dR = twoMq + 0.15052;
-        const double qE = q * EE, PR = dP * dR;
-        const double r_pair = 1. / (qE * PR);
-        const double inv_q = EE * (PR * r_pair);
-        const double inv_EE = q * (PR * r_pair);
-        const double inv_dP = dR * (qE * r_pair);
-        const double inv_dR = dP * (qE * r_pair);
+        double inv_q, inv_EE, inv_dP, inv_dR;
+        if constexpr (Safe) {
+            inv_q = 1. / q;
+            inv_EE = 1. / EE;
+            inv_dP = 1. / dP;
+            inv_dR = 1. / dR;
+        } else {  // the four over one reciprocal
+            const double qE = q * EE, PR = dP * dR;
+            const double r_pair = 1. / (qE * PR);
+            inv_q = EE * (PR * r_pair);
+            inv_EE = q * (PR * r_pair);
+            inv_dP = dR * (qE * r_pair);
+            inv_dR = dP * (qE * r_pair);
+        }
         const double inv_2Mq = inv_q * (0.5 / M);
         const double inv_l2 = 1. / ALLM_LAMBDA2;
 
         double ds = 0.;
 #pragma unroll 1
         for (int j = 0; j < 9; j++) {
-            const double Q2 = fm::exp(pQ2c + hd * GL9_X[j], t);
+            const double Q2 = nexp<Safe>(pQ2c + hd * GL9_X[j], t);
             // dcs_photonuclear_d2 (dcs.hh:335-355)
             const double x = Q2 * inv_2Mq;
             // dcs_photonuclear_f2_allm (dcs.hh:261-307)
-            const double tl = fm::log(fm::log((Q2 + ALLM_Q02) * inv_l2, t) * c.ph_inv_L0, t);
+            const double tl = nlog<Safe>(nlog<Safe>((Q2 + ALLM_Q02) * inv_l2, t) * c.ph_inv_L0, t);
             const double xP = (Q2 + 49.457) * inv_dP;
             const double xR = (Q2 + 0.15052) * inv_dR;
-            const double lnt = fm::log(tl, t);
-            const double gP = 1.0 + fm::exp(2.1979 * lnt, t), gA = 1.0 + fm::exp(1.1709 * lnt, t);
-            const double bP = 0.36292 + 1.8917 * fm::exp(1.8439 * lnt, t);
-            const double cR = 0.80107 + 0.97307 * fm::exp(3.4942 * lnt, t);
-            const double aR = 0.58400 + 0.37888 * fm::exp(2.6063 * lnt, t);
-            const double bR = 0.01147 + 3.7582 * fm::exp(0.49338 * lnt, t);
-            // dcs_photonuclear_r_whitlow (dcs.hh:322-332) as fractions:
-            // theta = (D_th + 12 c q2) / D_th, R = N_R / D_R, so
-            // 1 / (1 + R) = D_R D_th / S
+            // t -> 0 (Q^2 -> 0) gives log t = -inf and t^c = 0; clamped at -745
+            // every t^c still rounds to 0 against the constants it is added to
+            const double lnt = fmax(fm::log(tl, t), -745.0);
+            const double gP = 1.0 + nexp<Safe>(2.1979 * lnt, t), gA = 1.0 + nexp<Safe>(1.1709 * lnt, t);
+            const double bP = 0.36292 + 1.8917 * nexp<Safe>(1.8439 * lnt, t);
+            const double cR = 0.80107 + 0.97307 * nexp<Safe>(3.4942 * lnt, t);
+            const double aR = 0.58400 + 0.37888 * nexp<Safe>(2.6063 * lnt, t);
+            const double bR = 0.01147 + 3.7582 * nexp<Safe>(0.49338 * lnt, t);
+            // dcs_photonuclear_r_whitlow (dcs.hh:322-332)
             double q2 = Q2;
             if (Q2 < 0.3) q2 = 0.3;
-            const double lg = fm::log(q2 * 25.0, t);  // q2 / 0.04
+            const double lg = nlog<Safe>(q2 * 25.0, t);  // q2 / 0.04
             const double den = 0.09 + q2 * q2;
             const double Dth = (1.0 + q2) * (0.015625 + x * x);
-            const double DR = lg * q2 * den;
-            const double S = Dth * (DR + lg * (0.5747 * den - 0.3534 * q2)) +
-                             0.635 * q2 * den * (Dth + (12.0 * 0.015625) * q2);
-            // 1 / Q^2, 1 / (Q^2 + m0^2), 1 / (g_P g_A) and 1 / S over one reciprocal
             const double Qm = Q2 + 0.31985;
             const double gPA = gP * gA;
-            const double QQm = Q2 * Qm, gS = gPA * S;
-            const double r = 1.0 / (QQm * gS);
-            const double rq = gS * r, rg = QQm * r;
-            const double inv_Q2 = Qm * rq;
-            const double inv_Qm = Q2 * rq;
-            const double rG = S * rg;
-            const double inv_S = gPA * rg;
+            double inv_Q2, inv_Qm, rG, inv_1R;  // 1 / Q^2, 1 / (Q^2 + m0^2), 1 / (g_P g_A), 1 / (1 + R)
+            if constexpr (Safe) {
+                const double theta = 1. + (12.0 * 0.015625) * q2 / Dth;
+                const double R = 0.635 / lg * theta + 0.5747 / q2 - 0.3534 / den;
+                inv_1R = 1. / (1. + R);
+                inv_Q2 = 1. / Q2;
+                inv_Qm = 1. / Qm;
+                rG = 1. / gPA;
+            } else {
+                // theta = (D_th + 12 c q2) / D_th and R = N_R / D_R as fractions,
+                // so 1 / (1 + R) = D_R D_th / S; the four over one reciprocal
+                const double DR = lg * q2 * den;
+                const double S = Dth * (DR + lg * (0.5747 * den - 0.3534 * q2)) +
+                                 0.635 * q2 * den * (Dth + (12.0 * 0.015625) * q2);
+                const double QQm = Q2 * Qm, gS = gPA * S;
+                const double r = 1.0 / (QQm * gS);
+                const double rq = gS * r, rg = QQm * r;
+                inv_Q2 = Qm * rq;
+                inv_Qm = Q2 * rq;
+                rG = S * rg;
+                inv_1R = DR * Dth * (gPA * rg);
+            }
             const double cP = 0.28067 + (0.28067 - 0.22291) * (gA * rG - 1.0);
             const double aP = -0.0808 + (-0.0808 - -0.44812) * (gP * rG - 1.0);
-            const double l1x = fm::log(1. - x, t);
-            const double F2P = cP * fm::exp(aP * fm::log(xP, t) + bP * l1x, t);
-            const double F2R = cR * fm::exp(aR * fm::log(xR, t) + bR * l1x, t);
+            const double l1x = nlog<Safe>(1. - x, t);
+            const double F2P = cP * nexp<Safe>(aP * nlog<Safe>(xP, t) + bP * l1x, t);
+            const double F2R = cR * nexp<Safe>(aR * nlog<Safe>(xR, t) + bR * l1x, t);
             const double F2p = Q2 * inv_Qm * (F2P + F2R);
             // dcs_photonuclear_f2a_drss (dcs.hh:310-319)
             double a = 1.0;
             if (x < 0.0014)
                 a = c.ph_a_low;
             else if (x < 0.04)
-                a = fm::exp((0.069 * (fm::log(x, t) * INV_LN10) + 0.097) * c.ph_logA, t);
+                a = nexp<Safe>((0.069 * (nlog<Safe>(x, t) * INV_LN10) + 0.097) * c.ph_logA, t);
             const double F2A = c.ph_halfA * a * (2.0 + x * (-1.85 + x * (2.45 + x * (-2.35 + x)))) * F2p;
-            const double inv_1R = DR * Dth * inv_S;
             const double dds =
                 (1. - y + 0.5 * (1. - c.ph_2m2 * inv_Q2) * (y * y + Q2 * inv_EE) * inv_1R) * (inv_Q2 * inv_Q2) -
                 0.25 * inv_EE * inv_Q2;

@@ -141,6 +141,28 @@ NOA_HD inline double log(double x, const Tab& t) {
     return w + (r + std::fma(r * r, q, lo));
 }
 
+// log of a positive normal finite x, without the special-argument path
+// (~30% cheaper at a call site). Zero, negative, subnormal or infinite x
+// give an unspecified finite value: use it only where the argument is
+// bounded away from those by construction.
+NOA_HD inline double log_pos(double x, const Tab& t) {
+#pragma clang fp contract(off)
+    const uint64_t ix = as_bits(x);
+    const uint64_t tmp = ix - LOG_OFF;
+    const int i = int((tmp >> LOG_SHIFT) & uint64_t(LOG_N - 1));
+    const int k = int(int64_t(tmp) >> 52);
+    const double z = from_bits(ix - (tmp & 0xfff0000000000000ull));
+    const double kd = double(k);
+    const double r = std::fma(z, t.invc[i], -1.0);
+    const double w = fma_kb(kd, LN2_HI, t.lhi[i]);
+    double q = fma_kc(r, LOG_P4, LOG_P3);
+    q = fma_kc(q, r, LOG_P2);
+    q = fma_kc(q, r, LOG_P1);
+    q = fma_kc(q, r, LOG_P0);
+    const double lo = fma_kb(kd, LN2_LO, t.llo[i]);
+    return w + (r + std::fma(r * r, q, lo));
+}
+
 NOA_HD inline double exp(double x, const Tab& t) {
 #pragma clang fp contract(off)
     if (x != x) return x;
@@ -153,6 +175,30 @@ NOA_HD inline double exp(double x, const Tab& t) {
     r = fnma_kb(kd, LN2_N_LO, r);
     const int j = ki & (EXP_N - 1);
     const int m = ki >> 7;  // floor(ki / 128)
+    double p = fma_kc(r, 1.0 / 120.0, 1.0 / 24.0);
+    p = fma_kc(p, r, 1.0 / 6.0);
+    p = std::fma(p, r, 0.5);
+    p = std::fma(p, r, 1.0);
+    p = p * r;
+    const double th = t.ehi[j];
+    const double y = th + std::fma(th, p, t.elo[j]);
+    return std::ldexp(y, m);
+}
+
+// exp of a finite |x| < 1e12 (or NaN, which propagates), without the NaN
+// test and the clamp; overflow and underflow still come out of ldexp as inf,
+// 0 or subnormal. An infinite x gives NaN. k = rint(128 x / ln2) is read
+// from the low mantissa bits of 128 x / ln2 + 1.5 * 2^52 (no float-to-int
+// conversion, so no out-of-range case).
+NOA_HD inline double exp_fin(double x, const Tab& t) {
+#pragma clang fp contract(off)
+    const double shifted = x * INV_LN2_N + 0x1.8p52;
+    const int ki = int(uint32_t(as_bits(shifted)));
+    const double kd = shifted - 0x1.8p52;
+    double r = fnma_kb(kd, LN2_N_HI, x);
+    r = fnma_kb(kd, LN2_N_LO, r);
+    const int j = ki & (EXP_N - 1);
+    const int m = ki >> 7;
     double p = fma_kc(r, 1.0 / 120.0, 1.0 / 24.0);
     p = fma_kc(p, r, 1.0 / 6.0);
     p = std::fma(p, r, 0.5);

@@ -71,3 +71,21 @@ def test_f32mixed_error(el, device):
         assert r["finite"]
         assert r["zero_mismatch"] == 0, r
         assert r["max_rel"] <= F32_TOL, r
+
+
+def test_f64_special_inputs(device):
+    """NaN, infinite, zero, negative, subnormal and extreme (up to 1e300 GeV)
+    K and q: the FP64 kernel gives the oracle's values, NaNs and infinities
+    included (the integrals switch to their full-range path above 1e30 GeV)."""
+    vals = np.array([np.nan, np.inf, -np.inf, 0.0, -1.0, 1e-300, 5e-324, 1e300, 5e299, 1e200, 1e35, 1e29, 1e-3,
+                     0.5, 3.0, 10.0, 1e3, 1e6, 1e7])
+    K, q = (a.ravel().copy() for a in np.meshgrid(vals, vals))
+    with np.errstate(all="ignore"):
+        exp = oracle.dcs_map_all(K, q, STANDARD_ROCK, MUON_MASS, threads=0)
+    got = _gpu_all(K, q, STANDARD_ROCK, MUON_MASS, device)
+    for p in range(4):
+        g, e = got[p], exp[p]
+        same = (np.isnan(g) & np.isnan(e)) | (g == e)
+        fin = np.isfinite(g) & np.isfinite(e) & (e != 0)
+        same[fin] |= np.abs(g[fin] - e[fin]) <= 1e-10 * np.abs(e[fin])
+        assert same.all(), (p, K[~same][:5], q[~same][:5], g[~same][:5], e[~same][:5])
