@@ -38,7 +38,15 @@
 //   ionisation       src/noa/pms/dcs.hh:408-443
 #pragma once
 
+#if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
+#define NOA_DEV __device__
+#define NOA_DEV_INLINE __device__ __forceinline__
+#else  // host build of the same arithmetic, for the CPU tests
+#include <cmath>
+#define NOA_DEV
+#define NOA_DEV_INLINE inline
+#endif
 
 #include "dcs_consts.hh"
 #include "fastmath.hh"
@@ -53,14 +61,14 @@ namespace noa_hip {
 // literals kept as specified) and numerics.hh:137-144 (9 nodes on [-1,1]).
 constexpr double INV_LN10 = 0.43429448190325182765;  // 1 / ln 10
 
-static __constant__ double GL8_X[8] = {0.01985507, 0.10166676, 0.2372338, 0.40828268,
+NOA_FM_TABLE double GL8_X[8] = {0.01985507, 0.10166676, 0.2372338, 0.40828268,
                                 0.59171732, 0.7627662, 0.89833324, 0.98014493};
-static __constant__ double GL8_W[8] = {0.05061427, 0.11119052, 0.15685332, 0.18134189,
+NOA_FM_TABLE double GL8_W[8] = {0.05061427, 0.11119052, 0.15685332, 0.18134189,
                                 0.18134189, 0.15685332, 0.11119052, 0.05061427};
-static __constant__ double GL9_X[9] = {0.0000000000000000, -0.8360311073266358, 0.8360311073266358,
+NOA_FM_TABLE double GL9_X[9] = {0.0000000000000000, -0.8360311073266358, 0.8360311073266358,
                                 -0.9681602395076261, 0.9681602395076261, -0.3242534234038089,
                                 0.3242534234038089, -0.6133714327005904, 0.6133714327005904};
-static __constant__ double GL9_W[9] = {0.3302393550012598, 0.1806481606948574, 0.1806481606948574,
+NOA_FM_TABLE double GL9_W[9] = {0.3302393550012598, 0.1806481606948574, 0.1806481606948574,
                                 0.0812743883615744, 0.0812743883615744, 0.3123470770400029,
                                 0.3123470770400029, 0.2606106964029354, 0.2606106964029354};
 
@@ -69,7 +77,7 @@ struct F64 {
     static constexpr bool has_redo = false;
 
     // physics.hh:114-153
-    static __device__ __forceinline__ double brems(double K, double q, const DcsConsts& c, const fm::Tab& t) {
+    static NOA_DEV_INLINE double brems(double K, double q, const DcsConsts& c, const fm::Tab& t) {
 #pragma clang fp contract(off)
         const double me = ELECTRON_MASS;
         const double sqrte = SQRTE_BREMS;
@@ -91,7 +99,7 @@ struct F64 {
     }
 
     // dcs.hh:408-443
-    static __device__ __forceinline__ double ion(double K, double q, const DcsConsts& c, const fm::Tab& t) {
+    static NOA_DEV_INLINE double ion(double K, double q, const DcsConsts& c, const fm::Tab& t) {
 #pragma clang fp contract(off)
         const double me = ELECTRON_MASS;
         const double P2 = K * (K + c.i_2m);
@@ -112,7 +120,7 @@ struct F64 {
     }
 
     // dcs.hh:151-156 and 168-175: the exits before the integral.
-    static __device__ __forceinline__ bool pair_active(double K, double q, const DcsConsts& c) {
+    static NOA_DEV_INLINE bool pair_active(double K, double q, const DcsConsts& c) {
 #pragma clang fp contract(off)
         if (q <= 4. * ELECTRON_MASS) return false;
         if (q >= K + c.p_cut) return false;
@@ -126,11 +134,11 @@ struct F64 {
     // Node-loop log / exp: the full versions on the safe path, the ones
     // without special-argument handling otherwise.
     template <bool Safe>
-    static __device__ __forceinline__ double nlog(double x, const fm::Tab& t) {
+    static NOA_DEV_INLINE double nlog(double x, const fm::Tab& t) {
         return Safe ? fm::log(x, t) : fm::log_pos(x, t);
     }
     template <bool Safe>
-    static __device__ __forceinline__ double nexp(double x, const fm::Tab& t) {
+    static NOA_DEV_INLINE double nexp(double x, const fm::Tab& t) {
         return Safe ? fm::exp(x, t) : fm::exp_fin(x, t);
     }
 
@@ -140,13 +148,13 @@ struct F64 {
     static constexpr double FAST_E_MAX = 1e30;
 
     // dcs.hh:144-258 for a pair that passed pair_active.
-    static __device__ double pair(double K, double q, const DcsConsts& c, const fm::Tab& t) {
+    static NOA_DEV double pair(double K, double q, const DcsConsts& c, const fm::Tab& t) {
         if (K + c.mass <= FAST_E_MAX) return pair_impl<false>(K, q, c, t);
         return pair_impl<true>(K, q, c, t);  // also infinite and NaN K
     }
 
     template <bool Safe>
-    static __device__ double pair_impl(double K, double q, const DcsConsts& c, const fm::Tab& t) {
+    static NOA_DEV double pair_impl(double K, double q, const DcsConsts& c, const fm::Tab& t) {
 #pragma clang fp contract(off)
         const double me = ELECTRON_MASS;
         const double nu = q / (K + c.mass);
@@ -239,7 +247,7 @@ struct F64 {
     }
 
     // dcs.hh:357-381: the exits before the integral.
-    static __device__ __forceinline__ bool photo_active(double K, double q, const DcsConsts& c) {
+    static NOA_DEV_INLINE bool photo_active(double K, double q, const DcsConsts& c) {
 #pragma clang fp contract(off)
         if ((q < 1.) || (q < 2E-03 * K)) return false;
         const double E = K + c.mass;
@@ -251,13 +259,13 @@ struct F64 {
     }
 
     // dcs.hh:261-405 for a pair that passed photo_active.
-    static __device__ double photo(double K, double q, const DcsConsts& c, const fm::Tab& t) {
+    static NOA_DEV double photo(double K, double q, const DcsConsts& c, const fm::Tab& t) {
         if (K + c.mass <= FAST_E_MAX) return photo_impl<false>(K, q, c, t);
         return photo_impl<true>(K, q, c, t);
     }
 
     template <bool Safe>
-    static __device__ double photo_impl(double K, double q, const DcsConsts& c, const fm::Tab& t) {
+    static NOA_DEV double photo_impl(double K, double q, const DcsConsts& c, const fm::Tab& t) {
 #pragma clang fp contract(off)
         const double M = PHOTO_M;
         const double mpi = PHOTO_MPI;
@@ -363,10 +371,10 @@ struct F64 {
     }
 
     // The tile runner's interface; FP64 results are final (no recomputation).
-    static __device__ __forceinline__ double pair(double K, double q, const DcsConsts& c, bool&, const fm::Tab& t) {
+    static NOA_DEV_INLINE double pair(double K, double q, const DcsConsts& c, bool&, const fm::Tab& t) {
         return pair(K, q, c, t);
     }
-    static __device__ __forceinline__ double photo(double K, double q, const DcsConsts& c, bool&, const fm::Tab& t) {
+    static NOA_DEV_INLINE double photo(double K, double q, const DcsConsts& c, bool&, const fm::Tab& t) {
         return photo(K, q, c, t);
     }
 };
