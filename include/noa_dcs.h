@@ -212,18 +212,17 @@ typedef struct noa_dcs_surrogate_cfg {
 /* Surrogate DCS of the queries of an element batch (layout as in
  * noa_dcs_batch_create), FP32 outputs. Queries outside the training domain
  * and the seeded recheck sample take every process from the exact FP64
- * physics, other queries the processes flagged in their cell. fallback_idx
- * (int64) and fallback_el (uint32) are device workspaces of 2 x
- * (offsets[n_elements] - offsets[0]) entries; counters is a device array of
- * 4 uint32: on completion counters[1] and counters[3] = number of
+ * physics, other queries the processes flagged in their cell; the exact
+ * evaluations run in the same kernel. counters is a device array of 4
+ * uint32: on completion counters[1] and counters[3] = number of
  * pair-production and photonuclear evaluations done exactly, counters[2] =
  * bits of the largest relative error of the network over the recheck sample
  * (FP32, processes the network serves in that cell). Asynchronous on
  * `stream`; calls on one network are ordered by the stream. */
 int noa_dcs_surrogate_eval(noa_dcs_mlp* mlp, const noa_dcs_element* elements, const int64_t* offsets,
                            int32_t n_elements, double mass, const double* K, const double* q,
-                           const noa_dcs_outputs_f32* out, const noa_dcs_surrogate_cfg* cfg, int64_t* fallback_idx,
-                           uint32_t* fallback_el, uint32_t* counters, hipStream_t stream);
+                           const noa_dcs_outputs_f32* out, const noa_dcs_surrogate_cfg* cfg, uint32_t* counters,
+                           hipStream_t stream);
 
 /* ---- synthetic workload (counter-based, seeded) ----
  * u(seed, stream, i) is a 53-bit uniform in [0, 1) from a SplitMix64 output

@@ -98,8 +98,7 @@ SIGNATURES = {
                                          _P, _P]),
     "noa_dcs_mlp_infer_fused": (ctypes.c_int, [_P, _P, _I64, _P, _P]),
     "noa_dcs_surrogate_eval": (ctypes.c_int, [_P, ctypes.POINTER(Element), ctypes.POINTER(_I64), _I32, _D, _P, _P,
-                                              ctypes.POINTER(OutputsF32), ctypes.POINTER(SurrogateConfig), _P, _P,
-                                              _P, _P]),
+                                              ctypes.POINTER(OutputsF32), ctypes.POINTER(SurrogateConfig), _P, _P]),
 }
 
 _lib = None

@@ -12,7 +12,9 @@
 // say so) and routes to the exact FP64 path every query that is outside the
 // training domain, falls in a cell flagged by the validation error map, or
 // belongs to the seeded recheck sample; the recheck sample also measures the
-// surrogate's live error against the exact value.
+// surrogate's live error against the exact value. The exact integrals run in
+// the same kernel, compacted per work item in LDS as the map kernels do, so
+// routing costs no global lists or atomics.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -233,7 +235,18 @@ struct SurCfg {
     uint64_t seed;
 };
 
-constexpr uint32_t FB_RECHECK = 0x80000000u;  // the network's value is compared before the overwrite
+constexpr int SUR_ITEM = 4096;             // queries per work item: 8 passes of QWG
+constexpr uint32_t SUR_RECHECK = 0x8000u;  // list tag: compare the network's value before the overwrite
+
+// Per-item lists of the queries whose pair production / photonuclear DCS is
+// evaluated exactly (offset in the item, SUR_RECHECK tagged).
+struct SurLists {
+    uint16_t pair[SUR_ITEM];
+    uint16_t photo[SUR_ITEM];
+    uint32_t n_pair;
+    uint32_t n_photo;
+};
+static_assert(SUR_ITEM <= 0x8000 && SUR_ITEM % QWG == 0, "15-bit item offsets, whole passes");
 
 __device__ __forceinline__ uint64_t mix64(uint64_t z) {
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
@@ -250,42 +263,27 @@ __device__ __forceinline__ bool ion_active(double K, double q, const DcsConsts& 
     return !((Wmax < X_FRACTION * K) || (q > Wmax)) && !(q <= c.i_Wmin);
 }
 
-// Appends this lane's entry to a device list when `want`: one global atomic
-// per wave on counters[slot], entries at list[base + rank].
-__device__ __forceinline__ void wave_append(bool want, int64_t g, uint32_t tag, uint32_t* counters, int slot,
-                                            int64_t* __restrict__ idx, uint32_t* __restrict__ el) {
-    const unsigned long long m = __ballot(want);
-    if (!m) return;
-    const uint32_t rank = __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
-    uint32_t base = 0;
-    if (want && rank == 0) base = atomicAdd(&counters[slot], uint32_t(__popcll(m)));
-    const int leader = __ffsll((long long)m) - 1;
-    base = uint32_t(__builtin_amdgcn_readlane(int(base), leader));
-    if (want) {
-        idx[base + rank] = g;
-        el[base + rank] = tag;
-    }
-}
-
-// counters[0]: work-item counter, [1]: pair-production fallback entries (at
-// fb[0, cap)), [2]: recheck max rel error (float bits), [3]: photonuclear
-// fallback entries (at fb[cap, 2 cap)).
+// counters[0]: work-item counter, [1] / [3]: pair-production / photonuclear
+// DCS evaluated exactly, [2]: largest relative error of the network over the
+// recheck sample (float bits).
 //
-// Lane l owns query qb + l. Its routing comes first: out-of-domain queries
-// and the seeded recheck sample take every process exactly, others the
-// processes whose bit is set in the query's (element, log K, log nu) cell of
-// the flag map. A wave whose queries all need no network value skips the
-// network. Flagged bremsstrahlung and ionisation (closed forms) are
-// evaluated here; flagged active pair production and photonuclear queries
-// are appended to one dense list per process for fallback_kernel, so its
-// waves run one integral each.
+// A workgroup takes items of up to SUR_ITEM queries of one element. Phase A,
+// in passes of 64 queries per wave: lane l owns one query, whose routing
+// comes first (out-of-domain queries and the seeded recheck sample take every
+// process exactly, others the processes whose bit is set in the query's
+// (element, log K, log nu) cell of the flag map); a wave whose queries all
+// need no network value skips the network; flagged bremsstrahlung and
+// ionisation (closed forms) are evaluated in place, flagged open pair
+// production and photonuclear queries are appended to the item's lists in
+// LDS. Phase B drains the lists densely (photonuclear, the heavier, first),
+// so the FP64 integrals run in full waves.
 __global__ __launch_bounds__(IWG) void surrogate_dcs_kernel(
     const double* __restrict__ K, const double* __restrict__ q, const SurElem* __restrict__ els,
     const SurItem* __restrict__ items, uint32_t n_items, uint32_t* counters, Outs<float> o, SurCfg cfg,
-    int64_t* __restrict__ fb_idx, uint32_t* __restrict__ fb_el, int64_t cap, const uint16_t* __restrict__ frag,
-    const float* __restrict__ P) {
+    const uint16_t* __restrict__ frag, const float* __restrict__ P) {
     __shared__ MlpLds L;
     __shared__ double fm_block[fm::TABLE_DOUBLES];
+    __shared__ SurLists S;
     __shared__ uint32_t s_item;
     load_lds(L, frag, P);
     for (int k = threadIdx.x; k < fm::TABLE_DOUBLES; k += IWG) fm_block[k] = fm::table_entry(k);
@@ -293,156 +291,128 @@ __global__ __launch_bounds__(IWG) void surrogate_dcs_kernel(
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
     float worst = 0.f;
     for (;;) {
-        if (threadIdx.x == 0) s_item = atomicAdd(&counters[0], 1u);
+        if (threadIdx.x == 0) {
+            s_item = atomicAdd(&counters[0], 1u);
+            S.n_pair = 0;
+            S.n_photo = 0;
+        }
         __syncthreads();  // also publishes the LDS contents on the first pass
         const uint32_t it = __builtin_amdgcn_readfirstlane(s_item);
-        __syncthreads();  // every wave has read s_item before it is rewritten
         if (it >= n_items) break;  // workgroup-uniform exit
         const SurItem w = items[it];
         using ConstPtr = const __attribute__((address_space(4))) SurElem*;
         const SurElem& E = *(const SurElem*)((ConstPtr)(els + w.element));
         const int64_t end = w.begin + w.len;
-        const int64_t qb = w.begin + int64_t(wave) * QW;
-        if (qb >= end) continue;  // wave-uniform: this wave has no queries in this item
-        const int64_t g = qb + lane;
-        const bool live = g < end;
-        double k = 1., qq = 0.5;
-        uint32_t cellmask = 0, mask = 0;
-        bool sample = false;
-        float xs[IN] = {0.f, 0.f, 0.f};
-        if (live) {
-            k = K[g];
-            qq = q[g];
-            const double nu = qq / k;
-            if (!(k >= 1e-3 && k <= 1e6 && nu >= 1e-6 && nu < 1.0)) {
-                mask = 0xFu;  // out of the trained domain
-            } else {
-                const double lk = fm::log(k, tab) * INV_LN10, lnu = fm::log(nu, tab) * INV_LN10;
-                xs[0] = (float(lk) - LOGK_MID) / LOGK_HALF;
-                xs[1] = (float(lnu) - LOGNU_MID) / LOGNU_HALF;
-                xs[2] = float(E.c.Z) / 46.0f - 1.0f;
-                if (cfg.flag_map) {
-                    const int bins = cfg.bins;
-                    const int ik = min(bins - 1, max(0, int((lk + 3.0) / 9.0 * bins)));
-                    const int in = min(bins - 1, max(0, int((lnu + 6.0) / 6.0 * bins)));
-                    cellmask = cfg.flag_map[(int64_t(w.element) * bins + ik) * bins + in] & 0xFu;
+        // phase A
+#pragma unroll 1
+        for (int pass = 0; pass < SUR_ITEM / QWG; pass++) {
+            const int64_t qb = w.begin + int64_t(pass) * QWG + int64_t(wave) * QW;
+            if (qb >= end) break;  // wave-uniform: the later passes start further on
+            const int64_t g = qb + lane;
+            const bool live = g < end;
+            double k = 1., qq = 0.5;
+            uint32_t cellmask = 0, mask = 0;
+            bool sample = false;
+            float xs[IN] = {0.f, 0.f, 0.f};
+            if (live) {
+                k = K[g];
+                qq = q[g];
+                const double nu = qq / k;
+                if (!(k >= 1e-3 && k <= 1e6 && nu >= 1e-6 && nu < 1.0)) {
+                    mask = 0xFu;  // out of the trained domain
+                } else {
+                    const double lk = fm::log(k, tab) * INV_LN10, lnu = fm::log(nu, tab) * INV_LN10;
+                    xs[0] = (float(lk) - LOGK_MID) / LOGK_HALF;
+                    xs[1] = (float(lnu) - LOGNU_MID) / LOGNU_HALF;
+                    xs[2] = float(E.c.Z) / 46.0f - 1.0f;
+                    if (cfg.flag_map) {
+                        const int bins = cfg.bins;
+                        const int ik = min(bins - 1, max(0, int((lk + 3.0) / 9.0 * bins)));
+                        const int in = min(bins - 1, max(0, int((lnu + 6.0) / 6.0 * bins)));
+                        cellmask = cfg.flag_map[(int64_t(w.element) * bins + ik) * bins + in] & 0xFu;
+                    }
+                    sample = (mix64(cfg.seed ^ uint64_t(g)) % 1000u) < cfg.recheck_per_mille;
+                    mask = sample ? 0xFu : cellmask;
                 }
-                sample = (mix64(cfg.seed ^ uint64_t(g)) % 1000u) < cfg.recheck_per_mille;
-                mask = sample ? 0xFu : cellmask;
             }
-        }
-        const bool need_net = live && (mask != 0xFu || sample) && (mask != 0xFu || cellmask != 0xFu);
-        float t[OUT] = {0.f, 0.f, 0.f, 0.f};
-        if (__ballot(need_net)) {  // wave-uniform
-            // MFMA layout: column lane%32 of N-tile nt is query qb + 32 nt + lane%32
-            float x[2][IN];
+            const bool need_net = live && (mask != 0xFu || sample) && (mask != 0xFu || cellmask != 0xFu);
+            float t[OUT] = {0.f, 0.f, 0.f, 0.f};
+            if (__ballot(need_net)) {  // wave-uniform
+                // MFMA layout: column lane%32 of N-tile nt is query qb + 32 nt + lane%32
+                float x[2][IN];
 #pragma unroll
-            for (int j = 0; j < IN; j++) {
-                x[0][j] = __shfl(xs[j], lane & 31);
-                x[1][j] = __shfl(xs[j], 32 | (lane & 31));
-            }
-            float out[2][OUT];
-            mlp_wave(L, x, out);
-            // lane l takes query qb + l: N-tile 1's outputs move from lane l-32
-#pragma unroll
-            for (int p = 0; p < OUT; p++) {
-                const float o1 = __shfl(out[1][p], lane & 31);
-                t[p] = lane < 32 ? out[0][p] : o1;
-            }
-        }
-        bool to_pair = false, to_photo = false;
-        uint32_t tag_pair = uint32_t(w.element), tag_photo = uint32_t(w.element);
-        if (live) {
-            const bool act[OUT] = {true, F64::pair_active(k, qq, E.c), F64::photo_active(k, qq, E.c),
-                                   ion_active(k, qq, E.c)};
-            const float a = xs[0], b = xs[1], z = xs[2];
-            const float phi[NTREND] = {a,         b,         a * a, b * b, a * b, a * a * a, b * b * b,
-                                       a * a * b, a * b * b, z,     z * a, z * b, z * z};
-            float* dst[OUT] = {o.b, o.p, o.ph, o.i};
-#pragma unroll
-            for (int p = 0; p < OUT; p++) {
-                if (!dst[p]) continue;
-                float lnad = fmaf(cfg.stdv[p], t[p], cfg.mean[p]);
-#pragma unroll
-                for (int j = 0; j < NTREND; j++) lnad = fmaf(cfg.trend[p][j], phi[j], lnad);
-                const float net = __expf(lnad - E.ln_a);
-                const bool exact = (mask >> p) & 1u;
-                const bool recheck = sample && !((cellmask >> p) & 1u);
-                float v = net;
-                if (!act[p]) {
-                    v = 0.f;
-                } else if (exact && (p == 0 || p == 3)) {
-                    const double ex = (p == 0) ? F64::brems(k, qq, E.c, tab) : F64::ion(k, qq, E.c, tab);
-                    if (recheck && ex > 0.) worst = fmaxf(worst, float(fabs(double(net) / ex - 1.)));
-                    v = float(ex);
-                } else if (exact && p == 1) {
-                    to_pair = true;  // the network value stays for the recheck comparison
-                    if (recheck) tag_pair |= FB_RECHECK;
-                } else if (exact && p == 2) {
-                    to_photo = true;
-                    if (recheck) tag_photo |= FB_RECHECK;
+                for (int j = 0; j < IN; j++) {
+                    x[0][j] = __shfl(xs[j], lane & 31);
+                    x[1][j] = __shfl(xs[j], 32 | (lane & 31));
                 }
-                dst[p][g] = v;
+                float out[2][OUT];
+                mlp_wave(L, x, out);
+                // lane l takes query qb + l: N-tile 1's outputs move from lane l-32
+#pragma unroll
+                for (int p = 0; p < OUT; p++) {
+                    const float o1 = __shfl(out[1][p], lane & 31);
+                    t[p] = lane < 32 ? out[0][p] : o1;
+                }
             }
+            bool to_pair = false, to_photo = false;
+            uint32_t tag_pair = uint32_t(g - w.begin), tag_photo = tag_pair;
+            if (live) {
+                const bool act[OUT] = {true, F64::pair_active(k, qq, E.c), F64::photo_active(k, qq, E.c),
+                                       ion_active(k, qq, E.c)};
+                const float a = xs[0], b = xs[1], z = xs[2];
+                const float phi[NTREND] = {a,         b,         a * a, b * b, a * b, a * a * a, b * b * b,
+                                           a * a * b, a * b * b, z,     z * a, z * b, z * z};
+                float* dst[OUT] = {o.b, o.p, o.ph, o.i};
+#pragma unroll
+                for (int p = 0; p < OUT; p++) {
+                    if (!dst[p]) continue;
+                    float lnad = fmaf(cfg.stdv[p], t[p], cfg.mean[p]);
+#pragma unroll
+                    for (int j = 0; j < NTREND; j++) lnad = fmaf(cfg.trend[p][j], phi[j], lnad);
+                    const float net = __expf(lnad - E.ln_a);
+                    const bool exact = (mask >> p) & 1u;
+                    const bool recheck = sample && !((cellmask >> p) & 1u);
+                    float v = net;
+                    if (!act[p]) {
+                        v = 0.f;
+                    } else if (exact && (p == 0 || p == 3)) {
+                        const double ex = (p == 0) ? F64::brems(k, qq, E.c, tab) : F64::ion(k, qq, E.c, tab);
+                        if (recheck && ex > 0.) worst = fmaxf(worst, float(fabs(double(net) / ex - 1.)));
+                        v = float(ex);
+                    } else if (exact && p == 1) {
+                        to_pair = true;  // the network value stays for the recheck comparison
+                        if (recheck) tag_pair |= SUR_RECHECK;
+                    } else if (exact && p == 2) {
+                        to_photo = true;
+                        if (recheck) tag_photo |= SUR_RECHECK;
+                    }
+                    dst[p][g] = v;
+                }
+            }
+            wave_append(to_pair, &S.n_pair, S.pair, tag_pair);
+            wave_append(to_photo, &S.n_photo, S.photo, tag_photo);
         }
-        wave_append(to_pair, g, tag_pair, counters, 1, fb_idx, fb_el);
-        wave_append(to_photo, g, tag_photo, counters, 3, fb_idx + cap, fb_el + cap);
+        __syncthreads();  // the lists, and phase A's outputs for the recheck, are complete
+        // phase B: the item's exact integrals, densely
+        const uint32_t nph = S.n_photo, npa = S.n_pair;
+#pragma unroll 1
+        for (uint32_t v = threadIdx.x; v < nph + npa; v += IWG) {
+            const bool photo = v < nph;
+            const uint32_t tag = photo ? S.photo[v] : S.pair[v - nph];
+            const int64_t g = w.begin + (tag & (SUR_RECHECK - 1u));
+            const double k = K[g], qq = q[g];
+            const double val = photo ? F64::photo(k, qq, E.c, tab) : F64::pair(k, qq, E.c, tab);
+            float* dst = photo ? o.ph : o.p;
+            if ((tag & SUR_RECHECK) && val > 0.) worst = fmaxf(worst, float(fabs(double(dst[g]) / val - 1.)));
+            dst[g] = float(val);
+        }
+        if (threadIdx.x == 0) {
+            if (npa) atomicAdd(&counters[1], npa);
+            if (nph) atomicAdd(&counters[3], nph);
+        }
+        __syncthreads();  // the lists and s_item are rewritten by the next item
     }
     // non-negative floats order like their bit patterns
-    if (worst > 0.f) atomicMax(&counters[2], __float_as_uint(worst));
-}
-
-// Exact FP64 pair production (entries [0, counters[1])) and photonuclear
-// (entries [cap, cap + counters[3])) of the fallback lists; every entry is
-// kinematically active. A wave takes 64 consecutive entries of one list, so
-// it runs one integral; its lanes are grouped by element (the lists follow
-// query order, so a wave mostly holds one), and each group reads its
-// constants through scalar loads, as the map kernels do. For recheck
-// entries the network's value, already in the output, is compared first.
-__global__ __launch_bounds__(256) void fallback_kernel(const double* __restrict__ K, const double* __restrict__ q,
-                                                       const SurElem* __restrict__ els,
-                                                       const int64_t* __restrict__ fb_idx,
-                                                       const uint32_t* __restrict__ fb_el, int64_t cap,
-                                                       uint32_t* counters, Outs<float> o) {
-    __shared__ double fm_block[fm::TABLE_DOUBLES];
-    load_fm_tables(fm_block);
-    __syncthreads();
-    const fm::Tab tab = fm::tab_from_block(fm_block);
-    const int64_t n_pair = counters[1], n_photo = counters[3];
-    const int64_t w_pair = (n_pair + 63) / 64, w_all = w_pair + (n_photo + 63) / 64;
-    const int lane = threadIdx.x % 64;
-    const int64_t waves = int64_t(gridDim.x) * (blockDim.x / 64);
-    float worst = 0.f;
-    using ConstPtr = const __attribute__((address_space(4))) SurElem*;
-    for (int64_t w = int64_t(blockIdx.x) * (blockDim.x / 64) + threadIdx.x / 64; w < w_all; w += waves) {
-        const bool pair = w < w_pair;  // wave-uniform
-        const int64_t i = (pair ? w : w - w_pair) * 64 + lane;
-        bool todo = i < (pair ? n_pair : n_photo);
-        const int64_t slot = pair ? i : cap + i;
-        int64_t g = 0;
-        uint32_t tag = 0;
-        double k = 1., qq = 0.5;
-        if (todo) {
-            g = fb_idx[slot];
-            tag = fb_el[slot];
-            k = K[g];
-            qq = q[g];
-        }
-        const uint32_t el = tag & ~FB_RECHECK;
-        float* dst = pair ? o.p : o.ph;
-        for (;;) {
-            const unsigned long long m = __ballot(todo);
-            if (!m) break;
-            const uint32_t e = uint32_t(__builtin_amdgcn_readlane(int(el), __ffsll((long long)m) - 1));
-            const DcsConsts& c = ((const SurElem*)((ConstPtr)(els + e)))->c;
-            if (todo && el == e) {
-                const double v = pair ? F64::pair(k, qq, c, tab) : F64::photo(k, qq, c, tab);
-                if ((tag & FB_RECHECK) && v > 0.) worst = fmaxf(worst, float(fabs(double(dst[g]) / v - 1.)));
-                dst[g] = float(v);
-                todo = false;
-            }
-        }
-    }
     if (worst > 0.f) atomicMax(&counters[2], __float_as_uint(worst));
 }
 
@@ -490,8 +460,8 @@ int noa_dcs_mlp_infer_fused(noa_dcs_mlp* net, const float* X, int64_t n, float* 
 
 int noa_dcs_surrogate_eval(noa_dcs_mlp* net, const noa_dcs_element* elements, const int64_t* offsets,
                            int32_t n_elements, double mass, const double* K, const double* q,
-                           const noa_dcs_outputs_f32* out, const noa_dcs_surrogate_cfg* cfg, int64_t* fallback_idx,
-                           uint32_t* fallback_el, uint32_t* counters, hipStream_t s) {
+                           const noa_dcs_outputs_f32* out, const noa_dcs_surrogate_cfg* cfg, uint32_t* counters,
+                           hipStream_t s) {
     if (!net || !elements || !offsets || !out || !cfg || !counters) return fail(NOA_DCS_EINVAL, "NULL argument");
     if (n_elements < 1) return fail(NOA_DCS_EINVAL, "n_elements must be >= 1");
     if (offsets[0] < 0) return fail(NOA_DCS_EINVAL, "offsets[0] must be >= 0");
@@ -501,7 +471,7 @@ int noa_dcs_surrogate_eval(noa_dcs_mlp* net, const noa_dcs_element* elements, co
     }
     const int64_t total = offsets[n_elements] - offsets[0];
     if (total > 0xffffffffll) return fail(NOA_DCS_ERANGE, "at most 2^32 - 1 queries per call");
-    if (total > 0 && (!K || !q || !fallback_idx || !fallback_el)) return fail(NOA_DCS_EINVAL, "NULL argument");
+    if (total > 0 && (!K || !q)) return fail(NOA_DCS_EINVAL, "NULL argument");
     if (cfg->flag_map && (cfg->bins < 1 || cfg->bins > 4096)) return fail(NOA_DCS_EINVAL, "bins must be in [1, 4096]");
     if (cfg->recheck_per_mille > 1000) return fail(NOA_DCS_EINVAL, "recheck_per_mille must be <= 1000");
     for (int p = 0; p < OUT; p++)
@@ -516,8 +486,8 @@ int noa_dcs_surrogate_eval(noa_dcs_mlp* net, const noa_dcs_element* elements, co
     }
     std::vector<SurItem> items;
     for (int e = 0; e < n_elements; e++)
-        for (int64_t a = offsets[e]; a < offsets[e + 1]; a += QWG)
-            items.push_back(SurItem{a, int32_t(std::min<int64_t>(QWG, offsets[e + 1] - a)), e});
+        for (int64_t a = offsets[e]; a < offsets[e + 1]; a += SUR_ITEM)
+            items.push_back(SurItem{a, int32_t(std::min<int64_t>(SUR_ITEM, offsets[e + 1] - a)), e});
     const size_t need = els.size() * sizeof(SurElem) + items.size() * sizeof(SurItem) + 256;
     if (need > net->scratch_bytes) {
         if (net->scratch) (void)hipFree(net->scratch);
@@ -553,10 +523,7 @@ int noa_dcs_surrogate_eval(noa_dcs_mlp* net, const noa_dcs_element* elements, co
     if (!items.empty()) {
         const unsigned grid = unsigned(std::min<int64_t>(int64_t(items.size()), int64_t(n_cus())));
         hipLaunchKernelGGL(surrogate_dcs_kernel, dim3(grid), dim3(IWG), 0, s, K, q, d_els, d_items,
-                           uint32_t(items.size()), counters, o, c, fallback_idx, fallback_el, total, net->packed,
-                           net->params);
-        hipLaunchKernelGGL(fallback_kernel, dim3(unsigned(n_cus()) * 8), dim3(256), 0, s, K, q, d_els, fallback_idx,
-                           fallback_el, total, counters, o);
+                           uint32_t(items.size()), counters, o, c, net->packed, net->params);
     }
     if (int rc = check_hip(hipGetLastError(), "surrogate launch")) return rc;
     return ok();

@@ -248,9 +248,7 @@ class Surrogate:
         dev = K.device
         if outputs is None:
             outputs = tuple(torch.zeros(n, dtype=torch.float32, device=dev) for _ in range(NPROC))
-        if workspace is None:
-            workspace = self.workspace(n, dev)
-        fb_idx, fb_el, counters = workspace
+        counters = self.workspace(n, dev) if workspace is None else workspace
         els = (_lib.Element * len(self.elements))(*[_lib.Element(e.A, e.I, e.Z) for e in self.elements])
         offs = (ctypes.c_int64 * len(offsets))(*[int(x) for x in offsets])
         out = _lib.OutputsF32(*(_p(o) for o in outputs))
@@ -258,15 +256,13 @@ class Surrogate:
         with torch.cuda.device(dev):
             _lib.check(_lib.lib().noa_dcs_surrogate_eval(self.net.handle, els, offs, len(self.elements), float(mass),
                                                          _p(K), _p(q), ctypes.byref(out), ctypes.byref(cfg),
-                                                         _p(fb_idx), _p(fb_el), _p(counters), _stream(dev)))
+                                                         _p(counters), _stream(dev)))
         return outputs, counters
 
     @staticmethod
     def workspace(n: int, dev):
-        """Fallback lists (pair production and photonuclear, n entries each) and counters."""
-        return (torch.empty(2 * max(n, 1), dtype=torch.int64, device=dev),
-                torch.empty(2 * max(n, 1), dtype=torch.int32, device=dev),
-                torch.zeros(4, dtype=torch.int32, device=dev))
+        """The device counters of one evaluation (see read_counters)."""
+        return torch.zeros(4, dtype=torch.int32, device=dev)
 
     @staticmethod
     def read_counters(counters) -> tuple[int, float]:
@@ -479,7 +475,7 @@ def bench_entry(K, q, ref, steps, warmup, stream, wall_f64, tolerances=(None, 3e
             sur.evaluate(Kq, qq, off, outputs=outs, workspace=ws)
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t) / reps
-        c = Surrogate.read_counters_full(ws[2])
+        c = Surrogate.read_counters_full(ws)
         errs = max_rel_errors(outs, plan.outputs)
         res[f"tol_{tol:g}"] = {
             "evals_per_s": n / dt, "speedup_vs_f64": dt_exact / dt,

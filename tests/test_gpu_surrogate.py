@@ -94,7 +94,7 @@ def _n_active(ex, sl=slice(None)):
 def test_flagged_and_rechecked_queries_are_exact(device):
     """Routing: flagged processes, the recheck sample and out-of-domain
     queries come from the exact FP64 physics (pair production and
-    photonuclear through the dense fallback lists, one entry per active
+    photonuclear through the kernel's per-item exact lists, one entry per active
     pair); served processes from the network."""
     els = [STANDARD_ROCK]
     off = np.array([0, 20000])
