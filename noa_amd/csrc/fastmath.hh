@@ -123,10 +123,14 @@ NOA_HD inline double log(double x, const Tab& t) {
         ix = as_bits(x * 0x1p54);  // subnormal: scale into the normal range
         kadj = -54;
     }
-    const uint64_t tmp = ix - LOG_OFF;
-    const int i = int((tmp >> LOG_SHIFT) & uint64_t(LOG_N - 1));
-    const int k = int(int64_t(tmp) >> 52) + kadj;
-    const double z = from_bits(ix - (tmp & 0xfff0000000000000ull));
+    // LOG_OFF's low word is zero, so all the index and exponent work is on
+    // the high word (32-bit operations; a 64-bit shift into double(k) costs
+    // the compiler a two-word int64 conversion)
+    const uint32_t hx = uint32_t(ix >> 32);
+    const uint32_t ht = hx - uint32_t(LOG_OFF >> 32);
+    const int i = int((ht >> (LOG_SHIFT - 32)) & uint32_t(LOG_N - 1));
+    const int k = (int32_t(ht) >> 20) + kadj;
+    const double z = from_bits((uint64_t(hx - (ht & 0xfff00000u)) << 32) | (ix & 0xffffffffull));
     const double kd = double(k);
     const double r = std::fma(z, t.invc[i], -1.0);
     const double w = fma_kb(kd, LN2_HI, t.lhi[i]);  // exact
@@ -148,10 +152,11 @@ NOA_HD inline double log(double x, const Tab& t) {
 NOA_HD inline double log_pos(double x, const Tab& t) {
 #pragma clang fp contract(off)
     const uint64_t ix = as_bits(x);
-    const uint64_t tmp = ix - LOG_OFF;
-    const int i = int((tmp >> LOG_SHIFT) & uint64_t(LOG_N - 1));
-    const int k = int(int64_t(tmp) >> 52);
-    const double z = from_bits(ix - (tmp & 0xfff0000000000000ull));
+    const uint32_t hx = uint32_t(ix >> 32);
+    const uint32_t ht = hx - uint32_t(LOG_OFF >> 32);
+    const int i = int((ht >> (LOG_SHIFT - 32)) & uint32_t(LOG_N - 1));
+    const int k = int32_t(ht) >> 20;
+    const double z = from_bits((uint64_t(hx - (ht & 0xfff00000u)) << 32) | (ix & 0xffffffffull));
     const double kd = double(k);
     const double r = std::fma(z, t.invc[i], -1.0);
     const double w = fma_kb(kd, LN2_HI, t.lhi[i]);
