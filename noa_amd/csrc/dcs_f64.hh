@@ -166,6 +166,16 @@ struct F64 {
         const double argmin = (x0 + 2. * (1. - x0) * x1) / (1. + (1. - x1) * sqrt(1. - x0));
         const double tmin = fm::log(argmin, t);
 
+        // per-pair coefficients of the node polynomials in rho^2 (explicit
+        // FMAs: with contraction off the compiler keeps every product and
+        // sum of the reference's written order)
+        const double c1b = 1. + beta, c15b = 1. + 1.5 * beta, c2b = 1. + 2. * beta;
+        const double Ne0 = 5. + 4. * beta, Ne1 = 4. * beta - 1.;  // N_e = 5 - rho2 + 4 beta (1 + rho2)
+        const double Nm0 = 4. + 3. * beta, Nm1 = 1. + 3. * beta;  // N_mu = 4 + rho2 + 3 beta (1 + rho2)
+        const double De0 = -4. * beta, De1 = 2. * beta - 1., De2 = 2. * Nm1;  // D_e = De2 L - rho2 - 2 beta (2 - rho2)
+        const double Dm2 = 1.5 + 2. * beta;
+        const double mtw = -tmin;
+
         double I = 0.;
 #pragma unroll 1
         for (int j = 0; j < 8; j++) {
@@ -173,6 +183,7 @@ struct F64 {
             const double eps = nexp<Safe>(tj * tmin, t);
             const double rho = 1. - eps;
             const double rho2 = rho * rho;
+            const double opr2 = 1. + rho2;
             const double rho21 = eps * (2. - eps);
             const double xi = xi_factor * rho21;
             const double xi1 = 1. + xi;
@@ -180,10 +191,19 @@ struct F64 {
             // log(1 + 1/xi) - ... cancels ~3000-fold, so one ulp of 1/xi that
             // changes the rounding of 1 + 1/xi moves B_e by ~1e-10
             const double xi_i = 1. / xi;
-            const double De = 2. * (1. + 3. * beta) * nlog<Safe>(3. + xi_i, t) - rho2 - 2. * beta * (2. - rho2);
-            const double Dm = (1. + rho2) * (1.5 + 2. * beta) * nlog<Safe>(3. + xi, t) + 1. - 1.5 * rho2;
-            const double Ne = 5. - rho2 + 4. * beta * (1. + rho2);
-            const double Nm = 4. + rho2 + 3. * beta * (1. + rho2);
+            const double L3e = nlog<Safe>(3. + xi_i, t), L3m = nlog<Safe>(3. + xi, t);
+            double De, Dm, Ne, Nm;
+            if constexpr (Safe) {  // the reference's order (its overflow and NaN results)
+                De = 2. * (1. + 3. * beta) * L3e - rho2 - 2. * beta * (2. - rho2);
+                Dm = (1. + rho2) * (1.5 + 2. * beta) * L3m + 1. - 1.5 * rho2;
+                Ne = 5. - rho2 + 4. * beta * (1. + rho2);
+                Nm = 4. + rho2 + 3. * beta * (1. + rho2);
+            } else {
+                De = fma(De2, L3e, fma(rho2, De1, De0));
+                Dm = fma(opr2 * Dm2, L3m, fma(-1.5, rho2, 1.));
+                Ne = fma(rho2, Ne1, Ne0);
+                Nm = fma(rho2, Nm1, Nm0);
+            }
             double dr, Ye, Ymu;  // dr = (rho21 - beta) / (1 + xi), shared by B_e and B_mu
             if constexpr (Safe) {
                 dr = (rho21 - beta) / xi1;
@@ -196,29 +216,32 @@ struct F64 {
                 Ye = Ne * ((Dm * xi1) * r_y);
                 Ymu = Nm * ((De * xi1) * r_y);
             }
+            // B_e and B_mu keep the reference's order: near xi = 1e3 and 1e-3
+            // their terms cancel, and any reordering moves the result by up to
+            // ~4e-11 there
             double Be;
             if (xi >= 1E+03)
                 Be = 0.5 * xi_i * ((3. - rho2) + 2. * beta * (1. + rho2));
             else
-                Be = ((2. + rho2) * (1. + beta) + xi * (3. + rho2)) * nlog<Safe>(1. + xi_i, t) + dr - 3. - rho2;
+                Be = ((2. + rho2) * c1b + xi * (3. + rho2)) * nlog<Safe>(1. + xi_i, t) + dr - 3. - rho2;
             double Bmu;
             if (xi <= 1E-03)
                 Bmu = 0.5 * xi * (5. - rho2 + beta * (3. + rho2));
             else
-                Bmu = ((1. + rho2) * (1. + 1.5 * beta) - xi_i * (1. + 2. * beta) * rho21) * nlog<Safe>(1. + xi, t) +
-                      xi * dr + (1. + 2. * beta) * rho21;
+                Bmu = (opr2 * c15b - xi_i * c2b * rho21) * nlog<Safe>(1. + xi, t) + xi * dr + c2b * rho21;
             const double xe = xi1 * (1. + Ye);
             const double xmu = xi1 * (1. + Ymu);
             // q + (c_L / rho21) x = (q rho21 + c_L x) / rho21
             const double qr = q * rho21;
-            const double ge = qr + c.p_cL * xe;
-            const double Dle = (1. + c.p_cLe * xe) * (ge * ge);
-            const double Dlm = c.p_15Z13 * (qr + c.p_cL * xmu);
             double Le, Lmu;
             if constexpr (Safe) {
-                Le = fm::log(c.p_AZ13 * qr * sqrt(xe / Dle), t);
-                Lmu = fm::log(c.p_rAZ13 * qr / Dlm, t);
+                const double ge = qr + c.p_cL * xe;
+                Le = fm::log(c.p_AZ13 * qr * sqrt(xe / ((1. + c.p_cLe * xe) * (ge * ge))), t);
+                Lmu = fm::log(c.p_rAZ13 * qr / (c.p_15Z13 * (qr + c.p_cL * xmu)), t);
             } else {  // one reciprocal for the two screening denominators, no square root
+                const double ge = fma(c.p_cL, xe, qr);
+                const double Dle = fma(c.p_cLe, xe, 1.) * (ge * ge);
+                const double Dlm = c.p_15Z13 * fma(c.p_cL, xmu, qr);
                 const double r_l = 1. / (Dle * Dlm);
                 Le = 0.5 * fm::log_pos(c.p_AZ13_sq * (qr * qr) * xe * (Dlm * r_l), t);
                 Lmu = fm::log_pos(c.p_rAZ13 * qr * (Dle * r_l), t);
@@ -227,8 +250,14 @@ struct F64 {
             if (Phi_e < 0.) Phi_e = 0.;
             double Phi_mu = Bmu * Lmu;
             if (Phi_mu < 0.) Phi_mu = 0.;
-            const double f = -(Phi_e + Phi_mu * c.p_inv_r2) * (1. - rho) * tmin;
-            I += f * 1. * GL8_W[j];
+            if constexpr (Safe) {
+                const double f = -(Phi_e + Phi_mu * c.p_inv_r2) * (1. - rho) * tmin;
+                I += f * GL8_W[j];
+            } else {  // f = -(Phi_e + Phi_mu / r^2) (1 - rho) tmin; 1 - rho as the
+                // reference rounds it (not eps: they differ by ~1e-16 absolute,
+                // which is ~1e-11 relative where eps ~ 1e-5)
+                I = fma(fma(Phi_mu, c.p_inv_r2, Phi_e), (1. - rho) * (mtw * GL8_W[j]), I);
+            }
         }
 
         double zeta;
