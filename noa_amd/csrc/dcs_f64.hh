@@ -211,7 +211,7 @@ struct F64 {
                 Ymu = Nm / Dm;
             } else {  // one reciprocal for the three (D_e, D_mu > 1)
                 const double DD = De * Dm;
-                const double r_y = 1. / (DD * xi1);
+                const double r_y = fm::recip_pos(DD * xi1);
                 dr = (rho21 - beta) * (DD * r_y);
                 Ye = Ne * ((Dm * xi1) * r_y);
                 Ymu = Nm * ((De * xi1) * r_y);
@@ -242,7 +242,7 @@ struct F64 {
                 const double ge = fma(c.p_cL, xe, qr);
                 const double Dle = fma(c.p_cLe, xe, 1.) * (ge * ge);
                 const double Dlm = c.p_15Z13 * fma(c.p_cL, xmu, qr);
-                const double r_l = 1. / (Dle * Dlm);
+                const double r_l = fm::recip_pos(Dle * Dlm);
                 Le = 0.5 * fm::log_pos(c.p_AZ13_sq * (qr * qr) * xe * (Dlm * r_l), t);
                 Lmu = fm::log_pos(c.p_rAZ13 * qr * (Dle * r_l), t);
             }
@@ -320,7 +320,7 @@ struct F64 {
             inv_dR = 1. / dR;
         } else {  // the four over one reciprocal
             const double qE = q * EE, PR = dP * dR;
-            const double r_pair = 1. / (qE * PR);
+            const double r_pair = fm::recip_pos(qE * PR);
             inv_q = EE * (PR * r_pair);
             inv_EE = q * (PR * r_pair);
             inv_dP = dR * (qE * r_pair);
@@ -370,7 +370,7 @@ struct F64 {
                 const double S = Dth * (DR + lg * (0.5747 * den - 0.3534 * q2)) +
                                  0.635 * q2 * den * (Dth + (12.0 * 0.015625) * q2);
                 const double QQm = Q2 * Qm, gS = gPA * S;
-                const double r = 1.0 / (QQm * gS);
+                const double r = fm::recip_pos(QQm * gS);
                 const double rq = gS * r, rg = QQm * r;
                 inv_Q2 = Qm * rq;
                 inv_Qm = Q2 * rq;
