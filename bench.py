@@ -202,8 +202,9 @@ def consumers(dev, stream, steps: int) -> dict:
 def main() -> None:
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py runs on one GPU only: DCS evaluation needs no exchange between GPUs, and this build "
-              "measures the single-GPU kernel (see DESIGN.md for how the path shards).", file=sys.stderr)
+        print("bench.py runs on one GPU only: DCS evaluation needs no exchange between GPUs (each GPU would take a "
+              "contiguous slice of the pair arrays), so the measured figure is the single-GPU kernel.",
+              file=sys.stderr)
         sys.exit(2)
 
     import torch

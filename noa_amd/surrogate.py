@@ -1,5 +1,5 @@
-"""Learned surrogate of the four DCS processes, judged and backed by the exact
-FP64 kernel.
+"""Learned surrogate of the four DCS processes, validated against and backed by
+the exact FP64 kernel.
 
 Pipeline (all on the GPU):
   1. dataset: seeded (K, q) pairs over an element batch, exact DCS from the
