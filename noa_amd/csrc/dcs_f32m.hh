@@ -55,27 +55,22 @@ __device__ __forceinline__ float log1p_f(float u, const fm::Tab&) {
 // take s + s^2/2 (truncation s^3/6).
 __device__ __forceinline__ float expm1_f(float s, const fm::Tab& t) {
     if (fabsf(s) < 0x1p-20f) return fmaf(0.5f * s, s, s);
-    return float(fm::exp(double(s), t) - 1.0);
+    return float(fm::exp_fin(double(s), t) - 1.0);  // s finite: a node offset of the Q^2 range
 }
 
 // h(u) = (log1p(u) - u) / u^2 for u > 0, without the cancellation of the
-// direct form at small u: alternating series below 1/4 (12 terms, truncation
-// < 1e-8 relative), direct form above (cancellation factor < 8).
+// direct form at small u: below 1/4 a degree-6 polynomial (the Chebyshev
+// economization of the alternating series, from gen_fastmath_tables.py;
+// < 1e-8 relative before FP32 rounding), the direct form above
+// (cancellation factor < 8).
 __device__ __forceinline__ float log1p_h(float u, const fm::Tab& t) {
     if (u < 0.25f) {
-        float s = -1.0f / 13.0f;
-        s = fmaf(s, u, 1.0f / 12.0f);
-        s = fmaf(s, u, -1.0f / 11.0f);
-        s = fmaf(s, u, 1.0f / 10.0f);
-        s = fmaf(s, u, -1.0f / 9.0f);
-        s = fmaf(s, u, 1.0f / 8.0f);
-        s = fmaf(s, u, -1.0f / 7.0f);
-        s = fmaf(s, u, 1.0f / 6.0f);
-        s = fmaf(s, u, -1.0f / 5.0f);
-        s = fmaf(s, u, 1.0f / 4.0f);
-        s = fmaf(s, u, -1.0f / 3.0f);
-        s = fmaf(s, u, 1.0f / 2.0f);
-        return -s;
+        float s = fmaf(fm::H_P6, u, fm::H_P5);
+        s = fmaf(s, u, fm::H_P4);
+        s = fmaf(s, u, fm::H_P3);
+        s = fmaf(s, u, fm::H_P2);
+        s = fmaf(s, u, fm::H_P1);
+        return fmaf(s, u, fm::H_P0);
     }
     return fdiv(log1p_f(u, t) - u, u * u);
 }
@@ -187,7 +182,7 @@ struct F32M {
         for (int j = 0; j < 8; j++) {
             const float s = GL8_XF[j] * tmin;
             // eps = e^s and rho = 1 - eps from one FP64 table exp
-            const double ed = fm::exp(double(s), t);
+            const double ed = fm::exp_fin(double(s), t);  // s = t_j tmin, finite (tmin from argmin in (0, 1))
             const float eps = float(ed);
             const float rho = (fabsf(s) < 0x1p-20f) ? -fmaf(0.5f * s, s, s) : float(1.0 - ed);
             const float rho2 = rho * rho;

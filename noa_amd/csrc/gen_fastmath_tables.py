@@ -140,6 +140,39 @@ def log_poly(lo: Fraction, hi: Fraction, deg: int = LOG_DEG, terms: int = 40):
     return coef
 
 
+def h_poly(deg: int = 6, hi: Fraction = Fraction(1, 4), terms: int = 60):
+    """FP32 coefficients (constant first) of P(u) ~ h(u) = (log1p(u) - u) / u^2
+    on [0, hi] for the mixed-precision pair integrand: the series
+    sum_n (-1)^(n+1) u^n / (n + 2) to `terms` terms in the shifted and scaled
+    variable, its Chebyshev expansion cut after T_deg, back to powers of u;
+    coefficients rounded to float. Checked here: < 1e-8 relative over [0, hi]
+    in exact arithmetic (FP32 evaluation adds its own ~6e-8)."""
+    import struct
+    m, h = hi / 2, hi / 2
+    ps = [Fraction(0)] * (terms + 1)
+    for n in range(terms + 1):
+        cn = Fraction((-1) ** (n + 1), n + 2)
+        for k in range(n + 1):
+            ps[k] += cn * comb(n, k) * m ** (n - k) * h ** k
+    p_s = _chebyshev_to_power(_to_chebyshev(ps)[:deg + 1])
+    pr = [Fraction(0)] * (deg + 1)
+    for k, v in enumerate(p_s):
+        for j in range(k + 1):
+            pr[j] += v * comb(k, j) * (1 / h) ** k * (-m) ** (k - j)
+    coef = [struct.unpack("<f", struct.pack("<f", float(v)))[0] for v in pr]
+    worst = Decimal(0)
+    for i in range(1, 2001):
+        u = hi * Fraction(i, 2000)
+        ud = Decimal(u.numerator) / Decimal(u.denominator)
+        exact = ((1 + ud).ln() - ud) / (ud * ud)
+        v = Decimal(0)
+        for c in reversed(coef):
+            v = v * ud + Decimal(c)
+        worst = max(worst, abs((v - exact) / exact))
+    assert worst < Decimal("1e-8"), worst
+    return coef
+
+
 def exp_tables():
     th, tl = [], []
     for j in range(N_EXP):
@@ -193,6 +226,9 @@ def main() -> None:
     out.append("// log1p(r) = r + r^2 (LOG_P0 + r (LOG_P1 + ...)), r in [-2^-9, 2^-8]")
     for k, v in enumerate(poly):
         out.append(f"constexpr double LOG_P{k} = {float.hex(v)};")
+    out.append("// FP32: (log1p(u) - u) / u^2 = H_P0 + u (H_P1 + ...), u in [0, 1/4]")
+    for k, v in enumerate(h_poly()):
+        out.append(f"constexpr float H_P{k} = {float.hex(v)}f;")
     for name, vals in (("LOG_INVC", invc), ("LOG_LOGC_HI", lhi), ("LOG_LOGC_LO", llo),
                        ("EXP_T_HI", th), ("EXP_T_LO", tl)):
         out += ["", f"NOA_FM_TABLE double {name}[{len(vals)}] = {{", fmt(vals), "};"]
