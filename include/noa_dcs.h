@@ -150,6 +150,7 @@ typedef struct noa_dcs_batch_config {
     int32_t waves_per_cu;   /* persistent workgroups per CU; 0 = autotuned */
     int32_t use_graph;      /* 1 = capture the launch in a hipGraph */
     const char* tune_cache; /* autotune cache file, NULL = none */
+    int32_t unroll;         /* unroll of the integrals' node loops, 1 or 2; 0 = autotuned */
 } noa_dcs_batch_config;
 
 /* elements and offsets are HOST arrays (n_elements and n_elements + 1
@@ -160,8 +161,10 @@ int noa_dcs_batch_create(const noa_dcs_element* elements, const int64_t* offsets
                          void* out_photo, void* out_ion, const noa_dcs_batch_config* config,
                          hipStream_t stream, noa_dcs_batch** plan);
 int noa_dcs_batch_launch(noa_dcs_batch* plan, hipStream_t stream);
-/* Chosen tile and workgroups per CU, and the number of work items. */
-int noa_dcs_batch_info(const noa_dcs_batch* plan, int32_t* tile, int32_t* waves_per_cu, int64_t* n_items);
+/* Chosen tile, workgroups per CU and node-loop unroll, and the number of
+ * work items; any output pointer may be NULL. */
+int noa_dcs_batch_info(const noa_dcs_batch* plan, int32_t* tile, int32_t* waves_per_cu, int32_t* unroll,
+                       int64_t* n_items);
 int noa_dcs_batch_destroy(noa_dcs_batch* plan);
 
 /* ---- learned surrogate of the four DCS processes ----

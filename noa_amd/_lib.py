@@ -48,7 +48,7 @@ class OutputsF32(ctypes.Structure):
 class BatchConfig(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int32), ("tile", ctypes.c_int32),
                 ("waves_per_cu", ctypes.c_int32), ("use_graph", ctypes.c_int32),
-                ("tune_cache", ctypes.c_char_p)]
+                ("tune_cache", ctypes.c_char_p), ("unroll", ctypes.c_int32)]
 
 
 class SurrogateConfig(ctypes.Structure):
@@ -74,7 +74,8 @@ SIGNATURES = {
                                             _P, _P, _P, _P, ctypes.POINTER(BatchConfig), _P,
                                             ctypes.POINTER(_P)]),
     "noa_dcs_batch_launch": (ctypes.c_int, [_P, _P]),
-    "noa_dcs_batch_info": (ctypes.c_int, [_P, ctypes.POINTER(_I32), ctypes.POINTER(_I32), ctypes.POINTER(_I64)]),
+    "noa_dcs_batch_info": (ctypes.c_int, [_P, ctypes.POINTER(_I32), ctypes.POINTER(_I32), ctypes.POINTER(_I32),
+                                          ctypes.POINTER(_I64)]),
     "noa_dcs_batch_destroy": (ctypes.c_int, [_P]),
     "noa_dcs_fill_workload": (ctypes.c_int, [_P, _P, _I64, _U64, _U64, _D, _D, _D, _D, _P]),
     "noa_dcs_coulomb_data_f64": (ctypes.c_int, [_P, _I64, ctypes.POINTER(Element), _D, _P, _P, _P, _P, _P]),

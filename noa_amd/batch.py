@@ -23,7 +23,7 @@ _PREC = {"f64": _lib.PREC_F64, "f32mixed": _lib.PREC_F32MIXED}
 class BatchPlan:
     def __init__(self, elements, offsets, K: torch.Tensor, q: torch.Tensor, mass: float,
                  precision: str = "f64", outputs=None, tile: int = 0, waves_per_cu: int = 0,
-                 use_graph: bool = True, tune_cache: str | None = None):
+                 use_graph: bool = True, tune_cache: str | None = None, unroll: int = 0):
         if precision not in _PREC:
             raise ValueError(f"precision must be one of {list(_PREC)}")
         offsets = np.ascontiguousarray(offsets, dtype=np.int64)
@@ -48,7 +48,7 @@ class BatchPlan:
         els = (_lib.Element * len(elements))(*[_lib.Element(float(e.A), float(e.I), int(e.Z)) for e in elements])
         offs = (ctypes.c_int64 * len(offsets))(*[int(x) for x in offsets])
         cfg = _lib.BatchConfig(_PREC[precision], tile, waves_per_cu, int(use_graph),
-                               tune_cache.encode() if tune_cache else None)
+                               tune_cache.encode() if tune_cache else None, unroll)
         handle = ctypes.c_void_p()
         ptr = lambda t: None if t is None else ctypes.c_void_p(t.data_ptr())  # noqa: E731
         with torch.cuda.device(self.device):
@@ -66,9 +66,10 @@ class BatchPlan:
             _lib.check(_lib.lib().noa_dcs_batch_launch(self._handle, stream))
 
     def info(self) -> dict:
-        t, w, n = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int64()
-        _lib.check(_lib.lib().noa_dcs_batch_info(self._handle, ctypes.byref(t), ctypes.byref(w), ctypes.byref(n)))
-        return {"tile": t.value, "waves_per_cu": w.value, "n_items": n.value}
+        t, w, u, n = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int64()
+        _lib.check(_lib.lib().noa_dcs_batch_info(self._handle, ctypes.byref(t), ctypes.byref(w), ctypes.byref(u),
+                                                 ctypes.byref(n)))
+        return {"tile": t.value, "waves_per_cu": w.value, "unroll": u.value, "n_items": n.value}
 
     def close(self) -> None:
         if getattr(self, "_handle", None) is not None:

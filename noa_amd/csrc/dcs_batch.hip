@@ -73,6 +73,7 @@ struct noa_dcs_batch {
     int precision = 0;
     int tile = 0;
     int wpc = 0;
+    int unroll = 1;
     int use_graph = 1;
     int n_cu = 0;
     int device = 0;
@@ -116,7 +117,9 @@ hipError_t launch_tile(noa_dcs_batch* b, hipStream_t stream) {
 }
 
 hipError_t launch_direct(noa_dcs_batch* b, hipStream_t stream) {
-    return b->precision == NOA_DCS_PREC_F64 ? launch_tile<F64>(b, stream) : launch_tile<F32M>(b, stream);
+    if (b->precision == NOA_DCS_PREC_F64)
+        return b->unroll == 2 ? launch_tile<F64N<2>>(b, stream) : launch_tile<F64N<1>>(b, stream);
+    return b->unroll == 2 ? launch_tile<F32MN<2>>(b, stream) : launch_tile<F32MN<1>>(b, stream);
 }
 
 // Work items of `tile` pairs (the last of each element shorter), longest first.
@@ -152,24 +155,26 @@ std::string tune_key(const noa_dcs_batch* b) {
     int lg = 0;
     while ((int64_t(1) << (lg + 1)) <= total) lg++;
     char buf[256];
-    snprintf(buf, sizeof(buf), "%s|cu%d|prec%d|el%zu|log2n%d", prop.gcnArchName, b->n_cu, b->precision,
-             b->elements.size(), lg);
+    snprintf(buf, sizeof(buf), "%s|cu%d|prec%d|el%zu|log2n%d|tile-wpc-unroll", prop.gcnArchName, b->n_cu,
+             b->precision, b->elements.size(), lg);
     for (char* p = buf; *p; p++)
         if (*p == ' ') *p = '_';
     return buf;
 }
 
-bool cache_lookup(const char* path, const std::string& key, int* tile, int* wpc) {
+// One line per tuned configuration: key, tile, workgroups per CU, unroll.
+bool cache_lookup(const char* path, const std::string& key, int* tile, int* wpc, int* unroll) {
     if (!path) return false;
     FILE* f = fopen(path, "r");
     if (!f) return false;
-    char k[256];
-    int t, w;
+    char line[512], k[256];
+    int t, w, u;
     bool found = false;
-    while (fscanf(f, "%255s %d %d", k, &t, &w) == 3) {
-        if (key == k) {
+    while (fgets(line, sizeof(line), f)) {
+        if (sscanf(line, "%255s %d %d %d", k, &t, &w, &u) == 4 && key == k) {
             *tile = t;
             *wpc = w;
+            *unroll = u;
             found = true;
         }
     }
@@ -177,35 +182,40 @@ bool cache_lookup(const char* path, const std::string& key, int* tile, int* wpc)
     return found;
 }
 
-void cache_store(const char* path, const std::string& key, int tile, int wpc) {
+void cache_store(const char* path, const std::string& key, int tile, int wpc, int unroll) {
     if (!path) return;
     FILE* f = fopen(path, "a");
     if (!f) return;
-    fprintf(f, "%s %d %d\n", key.c_str(), tile, wpc);
+    fprintf(f, "%s %d %d %d\n", key.c_str(), tile, wpc, unroll);
     fclose(f);
 }
 
-// Times every (tile, workgroups per CU) candidate on the real batch and keeps
-// the fastest. The outputs are overwritten with correct values by each run.
-int autotune(noa_dcs_batch* b, hipStream_t stream, bool tune_tile, bool tune_wpc) {
+// Times every (tile, workgroups per CU, node-loop unroll) candidate on the
+// real batch and keeps the fastest. The outputs are overwritten with correct
+// values by each run.
+int autotune(noa_dcs_batch* b, hipStream_t stream, bool tune_tile, bool tune_wpc, bool tune_unroll) {
     const int tiles[3] = {1024, 2048, 4096};
     const int wpcs[4] = {2, 3, 4, 8};
-    const int fixed_tile = b->tile, fixed_wpc = b->wpc;
+    const int unrolls[2] = {1, 2};
+    const int fixed_tile = b->tile, fixed_wpc = b->wpc, fixed_unroll = b->unroll;
     hipEvent_t e0, e1;
     if (int rc = check_hip(hipEventCreate(&e0), "hipEventCreate")) return rc;
     if (int rc = check_hip(hipEventCreate(&e1), "hipEventCreate")) return rc;
     float best = 1e30f;
     int best_tile = tune_tile ? 2048 : fixed_tile, best_wpc = tune_wpc ? 4 : fixed_wpc;
+    int best_unroll = tune_unroll ? 1 : fixed_unroll;
     int rc = NOA_DCS_OK;
     for (int ti = 0; ti < 3 && rc == NOA_DCS_OK; ti++) {
         const int t = tiles[ti];
         if (!tune_tile && t != fixed_tile) continue;
         b->tile = t;
         if ((rc = upload_items(b))) break;
-        for (int wi = 0; wi < 4; wi++) {
-            const int w = wpcs[wi];
+        for (int wu = 0; wu < 8 && rc == NOA_DCS_OK; wu++) {
+            const int w = wpcs[wu % 4], u = unrolls[wu / 4];
             if (!tune_wpc && w != fixed_wpc) continue;
+            if (!tune_unroll && u != fixed_unroll) continue;
             b->wpc = w;
+            b->unroll = u;
             if ((rc = check_hip(launch_direct(b, stream), "batch warm-up launch"))) break;
             float ms_min = 1e30f;
             for (int r = 0; r < 3; r++) {
@@ -222,6 +232,7 @@ int autotune(noa_dcs_batch* b, hipStream_t stream, bool tune_tile, bool tune_wpc
                 best = ms_min;
                 best_tile = t;
                 best_wpc = w;
+                best_unroll = u;
             }
         }
     }
@@ -230,6 +241,7 @@ int autotune(noa_dcs_batch* b, hipStream_t stream, bool tune_tile, bool tune_wpc
     if (rc) return rc;
     b->tile = best_tile;
     b->wpc = best_wpc;
+    b->unroll = best_unroll;
     return upload_items(b);
 }
 
@@ -279,7 +291,7 @@ int noa_dcs_batch_create(const noa_dcs_element* elements, const int64_t* offsets
         if (int rc = check_element(&elements[e], mass)) return rc;
     }
     if (!K || !q) return fail(NOA_DCS_EINVAL, "K and q must be non-NULL");
-    const noa_dcs_batch_config def{NOA_DCS_PREC_F64, 0, 0, 1, nullptr};
+    const noa_dcs_batch_config def{NOA_DCS_PREC_F64, 0, 0, 1, nullptr, 0};
     const noa_dcs_batch_config& cfg = config ? *config : def;
     if (cfg.precision != NOA_DCS_PREC_F64 && cfg.precision != NOA_DCS_PREC_F32MIXED)
         return fail(NOA_DCS_EINVAL, "precision must be NOA_DCS_PREC_F64 or NOA_DCS_PREC_F32MIXED");
@@ -287,6 +299,8 @@ int noa_dcs_batch_create(const noa_dcs_element* elements, const int64_t* offsets
         return fail(NOA_DCS_EINVAL, "tile must be 0, 1024, 2048 or 4096 (got %d)", cfg.tile);
     if (cfg.waves_per_cu < 0 || cfg.waves_per_cu > 32)
         return fail(NOA_DCS_EINVAL, "waves_per_cu must be in [0, 32] (got %d)", cfg.waves_per_cu);
+    if (cfg.unroll < 0 || cfg.unroll > 2)
+        return fail(NOA_DCS_EINVAL, "unroll must be 0, 1 or 2 (got %d)", cfg.unroll);
 
     noa_dcs_batch* b = new noa_dcs_batch();
     b->precision = cfg.precision;
@@ -313,19 +327,22 @@ int noa_dcs_batch_create(const noa_dcs_element* elements, const int64_t* offsets
     if (!rc) {
         b->tile = cfg.tile ? cfg.tile : 2048;
         b->wpc = cfg.waves_per_cu ? cfg.waves_per_cu : 4;
-        const bool tune_tile = cfg.tile == 0, tune_wpc = cfg.waves_per_cu == 0;
-        if (tune_tile || tune_wpc) {
+        b->unroll = cfg.unroll ? cfg.unroll : 1;
+        const bool tune_tile = cfg.tile == 0, tune_wpc = cfg.waves_per_cu == 0, tune_unroll = cfg.unroll == 0;
+        if (tune_tile || tune_wpc || tune_unroll) {
             std::lock_guard<std::mutex> lock(g_tune_mutex);
             const std::string key = tune_key(b);
-            int t = 0, w = 0;
-            if (tune_tile && tune_wpc && cache_lookup(cfg.tune_cache, key, &t, &w) &&
-                (t == 1024 || t == 2048 || t == 4096) && w >= 1 && w <= 32) {
+            const bool all = tune_tile && tune_wpc && tune_unroll;
+            int t = 0, w = 0, u = 0;
+            if (all && cache_lookup(cfg.tune_cache, key, &t, &w, &u) && (t == 1024 || t == 2048 || t == 4096) &&
+                w >= 1 && w <= 32 && (u == 1 || u == 2)) {
                 b->tile = t;
                 b->wpc = w;
+                b->unroll = u;
                 rc = upload_items(b);
             } else {
-                rc = autotune(b, stream, tune_tile, tune_wpc);
-                if (!rc && tune_tile && tune_wpc) cache_store(cfg.tune_cache, key, b->tile, b->wpc);
+                rc = autotune(b, stream, tune_tile, tune_wpc, tune_unroll);
+                if (!rc && all) cache_store(cfg.tune_cache, key, b->tile, b->wpc, b->unroll);
             }
         } else {
             rc = upload_items(b);
@@ -352,10 +369,12 @@ int noa_dcs_batch_launch(noa_dcs_batch* plan, hipStream_t stream) {
     return ok();
 }
 
-int noa_dcs_batch_info(const noa_dcs_batch* plan, int32_t* tile, int32_t* waves_per_cu, int64_t* n_items) {
+int noa_dcs_batch_info(const noa_dcs_batch* plan, int32_t* tile, int32_t* waves_per_cu, int32_t* unroll,
+                       int64_t* n_items) {
     if (!plan) return fail(NOA_DCS_EINVAL, "plan is NULL");
     if (tile) *tile = plan->tile;
     if (waves_per_cu) *waves_per_cu = plan->wpc;
+    if (unroll) *unroll = plan->unroll;
     if (n_items) *n_items = plan->n_items;
     return ok();
 }

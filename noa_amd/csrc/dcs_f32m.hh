@@ -75,7 +75,9 @@ __device__ __forceinline__ float log1p_h(float u, const fm::Tab& t) {
     return fdiv(log1p_f(u, t) - u, u * u);
 }
 
-struct F32M {
+// NU: unroll factor of the node loops, as for F64N.
+template <int NU>
+struct F32MN {
     using out_t = float;
     static constexpr bool has_redo = true;
 
@@ -178,7 +180,7 @@ struct F32M {
 
         float I = 0.f;
         float err = 0.f;  // bound on the absolute FP32 error of I / |tmin|
-#pragma unroll 1
+#pragma unroll NU
         for (int j = 0; j < 8; j++) {
             const float s = GL8_XF[j] * tmin;
             // eps = e^s and rho = 1 - eps from one FP64 table exp
@@ -275,7 +277,7 @@ struct F32M {
         const float yy = y * y;
 
         float ds = 0.f;
-#pragma unroll 1
+#pragma unroll NU
         for (int j = 0; j < 9; j++) {
             const float s = hd * GL9_OMXF[j];  // ln(Q2max / Q2) >= 0
             // Q2 = Q2max e^-s; 2Mq - Q2 = c0 - Q2max expm1(-s) (= W^2 - M^2),
@@ -326,5 +328,7 @@ struct F32M {
         return fmaxf(r, 0.f);
     }
 };
+
+using F32M = F32MN<1>;
 
 }  // namespace noa_hip

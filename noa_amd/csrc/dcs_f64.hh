@@ -72,7 +72,10 @@ NOA_FM_TABLE double GL9_W[9] = {0.3302393550012598, 0.1806481606948574, 0.180648
                                 0.0812743883615744, 0.0812743883615744, 0.3123470770400029,
                                 0.3123470770400029, 0.2606106964029354, 0.2606106964029354};
 
-struct F64 {
+// NU: unroll factor of the integrals' node loops (1 or 2), a choice of the
+// batch autotuner; F64 is the plain form.
+template <int NU>
+struct F64N {
     using out_t = double;
     static constexpr bool has_redo = false;
 
@@ -177,7 +180,7 @@ struct F64 {
         const double mtw = -tmin;
 
         double I = 0.;
-#pragma unroll 1
+#pragma unroll NU
         for (int j = 0; j < 8; j++) {
             const double tj = GL8_X[j];
             const double eps = nexp<Safe>(tj * tmin, t);
@@ -330,7 +333,7 @@ struct F64 {
         const double inv_l2 = 1. / ALLM_LAMBDA2;
 
         double ds = 0.;
-#pragma unroll 1
+#pragma unroll NU
         for (int j = 0; j < 9; j++) {
             const double Q2 = nexp<Safe>(pQ2c + hd * GL9_X[j], t);
             // dcs_photonuclear_d2 (dcs.hh:335-355)
@@ -407,5 +410,7 @@ struct F64 {
         return photo(K, q, c, t);
     }
 };
+
+using F64 = F64N<1>;
 
 }  // namespace noa_hip

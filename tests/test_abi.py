@@ -87,9 +87,11 @@ def test_batch_argument_errors():
     assert create([0, 10, 20], cfg=_lib.BatchConfig(0, 1000, 0, 1, None)) == _lib.NOA_DCS_EINVAL
     assert "tile" in _lib.last_error()
     assert create([0, 10, 20], cfg=_lib.BatchConfig(0, 0, 99, 1, None)) == _lib.NOA_DCS_EINVAL
+    assert create([0, 10, 20], cfg=_lib.BatchConfig(0, 0, 0, 1, None, 3)) == _lib.NOA_DCS_EINVAL
+    assert "unroll" in _lib.last_error()
     assert plan.value is None
     assert lib.noa_dcs_batch_launch(None, None) == _lib.NOA_DCS_EINVAL
-    assert lib.noa_dcs_batch_info(None, None, None, None) == _lib.NOA_DCS_EINVAL
+    assert lib.noa_dcs_batch_info(None, None, None, None, None) == _lib.NOA_DCS_EINVAL
     assert lib.noa_dcs_batch_destroy(None) == _lib.NOA_DCS_OK
 
 

@@ -63,17 +63,36 @@ def test_batch_fixed_config_matches(small_batch, device):
     elements, offsets, Kh, qh, exp = small_batch
     K = torch.from_numpy(Kh).to(device)
     q = torch.from_numpy(qh).to(device)
-    ref = BatchPlan(elements, offsets, K, q, MUON_MASS, tile=2048, waves_per_cu=2, use_graph=False)
+    ref = BatchPlan(elements, offsets, K, q, MUON_MASS, tile=2048, waves_per_cu=2, use_graph=False, unroll=1)
     ref.launch()
-    for tile in (1024, 4096):
-        plan = BatchPlan(elements, offsets, K, q, MUON_MASS, tile=tile, waves_per_cu=3, use_graph=True)
+    for tile, unroll in ((1024, 1), (4096, 1), (2048, 2)):
+        plan = BatchPlan(elements, offsets, K, q, MUON_MASS, tile=tile, waves_per_cu=3, use_graph=True,
+                         unroll=unroll)
         plan.launch()
         torch.cuda.synchronize()
-        assert plan.info()["tile"] == tile
+        info = plan.info()
+        assert info["tile"] == tile and info["unroll"] == unroll, info
+        # the same arithmetic in every configuration: identical results
         for a, b in zip(plan.outputs, ref.outputs):
             assert torch.equal(a, b)
         plan.close()
     _check(ref, elements, offsets, exp, 1e-10)
+
+
+def test_batch_mixed_unroll_matches(small_batch, device):
+    elements, offsets, Kh, qh, exp = small_batch
+    K = torch.from_numpy(Kh).to(device)
+    q = torch.from_numpy(qh).to(device)
+    plans = [BatchPlan(elements, offsets, K, q, MUON_MASS, precision="f32mixed", tile=2048, waves_per_cu=4,
+                       unroll=u) for u in (1, 2)]
+    for p in plans:
+        p.launch()
+    torch.cuda.synchronize()
+    for a, b in zip(plans[0].outputs, plans[1].outputs):
+        assert torch.equal(a, b)
+    _check(plans[1], elements, offsets, exp, 1e-5)
+    for p in plans:
+        p.close()
 
 
 def test_device_batch_generator_matches_host(device):
