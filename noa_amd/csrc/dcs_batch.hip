@@ -39,11 +39,24 @@ struct WorkItem {
 
 __global__ void reset_counter_kernel(uint32_t* counter) { *counter = 0u; }
 
+// Minimum resident waves per SIMD asked of the compiler. Left free, the
+// item loop pushes the mixed form to 136 VGPRs (3 waves/SIMD); bounded to
+// 4 it fits 128 with no scratch. The unrolled FP64 form outgrows 256 VGPRs
+// (1 wave/SIMD, spilling into the accumulator registers); bounded to 2 or 4
+// it spills 1.1-1.6 KB per lane to scratch instead, so it stays unbounded.
+template <class P>
+struct BatchMinWaves {
+    static constexpr int value = 4;
+};
+template <>
+struct BatchMinWaves<F64N<2>> {
+    static constexpr int value = 1;
+};
+
 template <class P, int TILE>
-__global__ __launch_bounds__(WG) void dcs_batch_kernel(const double* __restrict__ K, const double* __restrict__ q,
-                                                       const DcsConsts* __restrict__ consts,
-                                                       const WorkItem* __restrict__ items, uint32_t n_items,
-                                                       uint32_t* counter, Outs<typename P::out_t> o) {
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(BatchMinWaves<P>::value))) void dcs_batch_kernel(
+    const double* __restrict__ K, const double* __restrict__ q, const DcsConsts* __restrict__ consts,
+    const WorkItem* __restrict__ items, uint32_t n_items, uint32_t* counter, Outs<typename P::out_t> o) {
     __shared__ TileLists<TILE> s;
     __shared__ uint32_t s_item;
     __shared__ double fm_block[fm::TABLE_DOUBLES];
