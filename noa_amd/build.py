@@ -84,6 +84,17 @@ def build_extension(verbose: bool = False) -> str:
     return out
 
 
+def build_test_kernels() -> str:
+    """tests/native/libfm_dev.so: the device log / exp / reciprocal of
+    csrc/fastmath.hh over arrays, for the GPU accuracy tests (not part of the
+    product library)."""
+    native = os.path.join(ROOT, "tests", "native")
+    out = os.path.join(native, "libfm_dev.so")
+    _run([HIPCC, *HIP_FLAGS, "-shared", "-o", out + ".tmp", os.path.join(native, "fastmath_dev.hip")])
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_oracle(with_reference: bool = True) -> None:
     oracle = os.path.join(ROOT, "oracle")
     _run(["make", "-C", oracle, "all", f"PY={sys.executable}"])
@@ -95,6 +106,7 @@ def main() -> None:
     verbose = "-v" in sys.argv
     print("building", build_library(verbose), flush=True)
     print("building", build_extension(verbose), flush=True)
+    print("building", build_test_kernels(), flush=True)
     build_oracle()
     print("oracle built", flush=True)
 

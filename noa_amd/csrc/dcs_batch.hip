@@ -213,7 +213,10 @@ int autotune(noa_dcs_batch* b, hipStream_t stream, bool tune_tile, bool tune_wpc
     const int fixed_tile = b->tile, fixed_wpc = b->wpc, fixed_unroll = b->unroll;
     hipEvent_t e0, e1;
     if (int rc = check_hip(hipEventCreate(&e0), "hipEventCreate")) return rc;
-    if (int rc = check_hip(hipEventCreate(&e1), "hipEventCreate")) return rc;
+    if (int rc = check_hip(hipEventCreate(&e1), "hipEventCreate")) {
+        (void)hipEventDestroy(e0);
+        return rc;
+    }
     float best = 1e30f;
     int best_tile = tune_tile ? 2048 : fixed_tile, best_wpc = tune_wpc ? 4 : fixed_wpc;
     int best_unroll = tune_unroll ? 1 : fixed_unroll;

@@ -192,8 +192,12 @@ struct F64N {
             const double xi1 = 1. + xi;
             // correctly rounded, as in the reference: near xi = 1e3, B_e's
             // log(1 + 1/xi) - ... cancels ~3000-fold, so one ulp of 1/xi that
-            // changes the rounding of 1 + 1/xi moves B_e by ~1e-10
-            const double xi_i = 1. / xi;
+            // changes the rounding of 1 + 1/xi moves B_e by ~1e-10. recip_pos
+            // ends in a Newton step from an approximation within 2^-46, so it
+            // can round differently from 1 / xi only where 1 / xi lies within
+            // ~2^-92 relative of a rounding midpoint (~2^-39 of inputs); xi >=
+            // 1e-66 here, so xi is positive normal
+            const double xi_i = Safe ? 1. / xi : fm::recip_pos(xi);
             const double L3e = nlog<Safe>(3. + xi_i, t), L3m = nlog<Safe>(3. + xi, t);
             double De, Dm, Ne, Nm;
             if constexpr (Safe) {  // the reference's order (its overflow and NaN results)
