@@ -59,7 +59,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(BatchMinWave
     const WorkItem* __restrict__ items, uint32_t n_items, uint32_t* counter, Outs<typename P::out_t> o) {
     __shared__ TileLists<TILE> s;
     __shared__ uint32_t s_item;
-    __shared__ double fm_block[fm::TABLE_DOUBLES];
+    __shared__ __attribute__((aligned(fm::TABLE_LDS_ALIGN))) double fm_block[fm::TABLE_DOUBLES];
     load_fm_tables(fm_block);  // published by the barrier below
     const fm::Tab tab = fm::tab_from_block(fm_block);
     for (;;) {

@@ -342,10 +342,10 @@ __device__ __forceinline__ double photo_ratio(double K, double q, double mass, c
 
 // ---- kernels ----
 
-#define NOA_FM_PROLOGUE()                          \
-    __shared__ double fm_block[fm::TABLE_DOUBLES]; \
-    load_fm_tables(fm_block);                      \
-    __syncthreads();                               \
+#define NOA_FM_PROLOGUE()                                                                     \
+    __shared__ __attribute__((aligned(fm::TABLE_LDS_ALIGN))) double fm_block[fm::TABLE_DOUBLES]; \
+    load_fm_tables(fm_block);                                                                 \
+    __syncthreads();                                                                          \
     const fm::Tab tab = fm::tab_from_block(fm_block)
 
 #define NOA_GRID_LOOP(i, n) \

@@ -48,7 +48,7 @@ __device__ __forceinline__ double node_t(double K, const RecoilGrid& g, int j, c
 
 __global__ __launch_bounds__(256) void recoil_nodes_kernel(const double* __restrict__ K, int64_t nK, RecoilGrid g,
                                                            double* __restrict__ Kn, double* __restrict__ qn) {
-    __shared__ double fm_block[fm::TABLE_DOUBLES];
+    __shared__ __attribute__((aligned(fm::TABLE_LDS_ALIGN))) double fm_block[fm::TABLE_DOUBLES];
     load_fm_tables(fm_block);
     __syncthreads();
     const fm::Tab tab = fm::tab_from_block(fm_block);
@@ -93,7 +93,7 @@ __device__ __forceinline__ double analytic_ionisation(double K, double xlow, con
 __global__ __launch_bounds__(256) void recoil_reduce_kernel(const double* __restrict__ K, int64_t nK, RecoilGrid g,
                                                             const double* __restrict__ qn, Outs<double> dcs,
                                                             DcsConsts c, Outs<double> del, Outs<double> cel) {
-    __shared__ double fm_block[fm::TABLE_DOUBLES];
+    __shared__ __attribute__((aligned(fm::TABLE_LDS_ALIGN))) double fm_block[fm::TABLE_DOUBLES];
     load_fm_tables(fm_block);
     __syncthreads();
     const fm::Tab tab = fm::tab_from_block(fm_block);

@@ -45,6 +45,10 @@ namespace fm {
 
 constexpr uint64_t LOG_OFF = 0x3fe6000000000000ull;  // bits of 0.6875
 constexpr int TABLE_DOUBLES = 3 * LOG_N + 2 * EXP_N;
+// LDS alignment of the table block: ds_read2st64_b64 reads two entries 64 x 8
+// bytes apart at one address with offsets in 512-byte units, so at a 512-byte
+// aligned base the byte index i * 8 is the whole address (no add per call).
+constexpr int TABLE_LDS_ALIGN = 512;
 
 // Pointers to one copy of the tables (LDS in the kernels).
 struct Tab {

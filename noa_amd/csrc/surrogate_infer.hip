@@ -282,7 +282,7 @@ __global__ __launch_bounds__(IWG) void surrogate_dcs_kernel(
     const SurItem* __restrict__ items, uint32_t n_items, uint32_t* counters, Outs<float> o, SurCfg cfg,
     const uint16_t* __restrict__ frag, const float* __restrict__ P) {
     __shared__ MlpLds L;
-    __shared__ double fm_block[fm::TABLE_DOUBLES];
+    __shared__ __attribute__((aligned(fm::TABLE_LDS_ALIGN))) double fm_block[fm::TABLE_DOUBLES];
     __shared__ SurLists S;
     __shared__ uint32_t s_item;
     load_lds(L, frag, P);
