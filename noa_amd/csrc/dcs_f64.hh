@@ -254,9 +254,15 @@ struct F64N {
                 Lmu = fm::log_pos(c.p_rAZ13 * qr * (Dle * r_l), t);
             }
             double Phi_e = Be * Le;
-            if (Phi_e < 0.) Phi_e = 0.;
             double Phi_mu = Bmu * Lmu;
-            if (Phi_mu < 0.) Phi_mu = 0.;
+            if constexpr (Safe) {
+                if (Phi_e < 0.) Phi_e = 0.;
+                if (Phi_mu < 0.) Phi_mu = 0.;
+            } else {  // one v_max each; a NaN here comes from a NaN q, which
+                // the (E - q) / q factor of the result carries anyway
+                Phi_e = fmax(Phi_e, 0.);
+                Phi_mu = fmax(Phi_mu, 0.);
+            }
             if constexpr (Safe) {
                 const double f = -(Phi_e + Phi_mu * c.p_inv_r2) * (1. - rho) * tmin;
                 I += f * GL8_W[j];
@@ -356,7 +362,11 @@ struct F64N {
             const double bR = 0.01147 + 3.7582 * nexp<Safe>(0.49338 * lnt, t);
             // dcs_photonuclear_r_whitlow (dcs.hh:322-332)
             double q2 = Q2;
-            if (Q2 < 0.3) q2 = 0.3;
+            if constexpr (Safe) {
+                if (Q2 < 0.3) q2 = 0.3;
+            } else {  // a NaN Q^2 (NaN q) still reaches the result through Q^2 itself
+                q2 = fmax(Q2, 0.3);
+            }
             const double lg = nlog<Safe>(q2 * 25.0, t);  // q2 / 0.04
             const double den = 0.09 + q2 * q2;
             const double Dth = (1.0 + q2) * (0.015625 + x * x);

@@ -214,11 +214,13 @@ NOA_HD inline double exp(double x, const Tab& t) {
 // exp of a finite |x| < 1e12 (or NaN, which propagates), without the NaN
 // test and the clamp; overflow and underflow still come out of ldexp as inf,
 // 0 or subnormal. An infinite x gives NaN. k = rint(128 x / ln2) is read
-// from the low mantissa bits of 128 x / ln2 + 1.5 * 2^52 (no float-to-int
-// conversion, so no out-of-range case).
+// from the low mantissa bits of 128 x / ln2 + 1.5 * 2^52, formed by one FMA
+// (no float-to-int conversion, so no out-of-range case; k may differ from
+// the rounded product's at a tie, which leaves |r| within ln2 / 256 + 1 ulp,
+// far inside the polynomial's range).
 NOA_HD inline double exp_fin(double x, const Tab& t) {
 #pragma clang fp contract(off)
-    const double shifted = x * INV_LN2_N + 0x1.8p52;
+    const double shifted = std::fma(x, INV_LN2_N, 0x1.8p52);
     const int ki = int(uint32_t(as_bits(shifted)));
     const double kd = shifted - 0x1.8p52;
     double r = fnma_kb(kd, LN2_N_HI, x);
