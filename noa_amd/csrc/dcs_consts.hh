@@ -47,6 +47,13 @@ struct DcsConsts {
     double f_pair_c;   // 1.794664e-34 * Z * 1e3 * N_A / A
     double f_photo_c;  // 2.603096e-35 * 0.5 * 1e3 * N_A / A
     double f_ion_c;    // 1.535336e-5 * Z / A
+    // correctly rounded reciprocals for the quotients of the FP64 closed
+    // forms (the kernels divide through them with a Markstein step)
+    double r_A, r_mass, i_r_m2;  // 1 / A, 1 / mass, 1 / mass^2
+    // 1 when mass and A lie where every closed-form quotient of an ordinary
+    // (K, q) stays in the normal range (1e-6 <= mass <= 1e6 GeV, 1 <= A <=
+    // 1e4 g/mol); otherwise the kernels keep the reference's divisions
+    int32_t fast_ok, pad_;
 };
 
 // The same arithmetic, in the same order, as the reference's inline
@@ -117,6 +124,10 @@ inline DcsConsts make_consts(double A, double I, int32_t Z, double mass) {
     c.f_pair_c = c.p_dcs_c * 1E+03 * AVOGADRO_NUMBER / A;
     c.f_photo_c = 2.603096E-35 * 0.5 * 1E+03 * AVOGADRO_NUMBER / A;
     c.f_ion_c = 1.535336E-05 * double(Z) / A;
+    c.r_A = 1. / A;
+    c.r_mass = 1. / mass;
+    c.i_r_m2 = 1. / c.i_m2;
+    c.fast_ok = (mass >= 1e-6 && mass <= 1e6 && A >= 1. && A <= 1e4) ? 1 : 0;
     return c;
 }
 

@@ -147,7 +147,7 @@ struct F32MN {
     }
 
     static __device__ __forceinline__ bool pair_active(double K, double q, const DcsConsts& c) {
-        return F64::pair_active(K, q, c);
+        return F64::pair_active_impl<true>(K, q, c);
     }
 
     // Sets `redo` when the running bound on the FP32 rounding of the two
@@ -245,7 +245,7 @@ struct F32MN {
     }
 
     static __device__ __forceinline__ bool photo_active(double K, double q, const DcsConsts& c) {
-        return F64::photo_active(K, q, c);
+        return F64::photo_active_impl<true>(K, q, c);
     }
 
     static __device__ float photo(double K, double q, const DcsConsts& c, bool& redo, const fm::Tab& t) {

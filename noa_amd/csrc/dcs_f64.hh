@@ -29,6 +29,12 @@
 // arithmetic around them. Above 1e30 GeV, and for infinite or NaN K, the
 // integrals run with the full log / exp (the same code, templated), so the
 // reference's overflow results are kept.
+// The quotients of the closed forms, the exits and the integrals' set-up
+// (a / b in the reference) are fm::div_rn(a, b, 1 / b): a correctly rounded
+// reciprocal refined by one Markstein step, which rounds as the division
+// does for the normal operands that `ordinary` (K, q) and the element's
+// fast_ok flag guarantee, at about a third of the IEEE sequence's cost; other
+// inputs take the same code with the divisions (the Safe instantiations).
 // Otherwise the device log/exp (the table-driven ones of fastmath.hh, ~0.5
 // ulp) against glibc is the only difference from the reference's CPU path.
 // Launch invariants come precomputed in DcsConsts.
@@ -79,58 +85,120 @@ struct F64N {
     using out_t = double;
     static constexpr bool has_redo = false;
 
+    // Quotients of the closed forms. On the fast path a / b is div_rn(a, b,
+    // 1 / b) with 1 / b from recip_nz, both rounding as the IEEE division
+    // does, so the results (and every exit decision taken on them) are the
+    // reference's; the Safe instantiation divides. `ordinary` selects the
+    // fast path per lane: 1e-20 <= K <= 1e30 and 1e-30 <= q <= 1e30 GeV
+    // (NaN fails), for an element whose constants allow it. Under those
+    // bounds every divisor and quotient below is normal (or an exact zero
+    // numerator), which div_rn needs.
+    template <bool Safe>
+    static NOA_DEV_INLINE double rcp(double b) {
+        if constexpr (Safe) return 1. / b;
+        else return fm::recip_nz(b);
+    }
+    template <bool Safe>
+    static NOA_DEV_INLINE double dv(double a, double b, double rb) {
+        if constexpr (Safe) return a / b;
+        else return fm::div_rn(a, b, rb);
+    }
+    static NOA_DEV_INLINE bool ordinary(double K, double q, const DcsConsts& c) {
+        return c.fast_ok && (K >= 1e-20) && (K <= FAST_E_MAX) && (q >= 1e-30) && (q <= FAST_E_MAX);
+    }
+    static constexpr double R_ME = 1. / ELECTRON_MASS;  // correctly rounded at compile time
+
     // physics.hh:114-153
     static NOA_DEV_INLINE double brems(double K, double q, const DcsConsts& c, const fm::Tab& t) {
+#pragma clang fp contract(off)
+        // 1 - nu >= 2^-20 keeps delta's divisor normal on the fast path
+        if (ordinary(K, q, c) && q <= (K + c.mass) * (1. - 0x1p-20)) return brems_impl<false>(K, q, c, t);
+        return brems_impl<true>(K, q, c, t);
+    }
+
+    template <bool Safe>
+    static NOA_DEV_INLINE double brems_impl(double K, double q, const DcsConsts& c, const fm::Tab& t) {
 #pragma clang fp contract(off)
         const double me = ELECTRON_MASS;
         const double sqrte = SQRTE_BREMS;
         const double E = K + c.mass;
-        const double delta_factor = c.b_half_m2 / E;
-        const double qe_max = E / (1. + c.b_half_m2 / (me * E));
-        const double nu = q / E;
-        const double delta = delta_factor * nu / (1. - nu);
-        double Phi_n = fm::log(c.b_BZ_n * (c.mass + delta * c.b_Dn_sqrte_m2) /
-                           (c.b_D_n * (me + delta * sqrte * c.b_BZ_n)), t);
+        const double rE = Safe ? 0. : rcp<Safe>(E);
+        const double delta_factor = dv<Safe>(c.b_half_m2, E, rE);
+        const double meE = me * E;
+        const double t1 = dv<Safe>(c.b_half_m2, meE, Safe ? 0. : rcp<Safe>(meE));
+        const double d1 = 1. + t1;
+        const double qe_max = dv<Safe>(E, d1, Safe ? 0. : rcp<Safe>(d1));
+        const double nu = dv<Safe>(q, E, rE);
+        const double omn = 1. - nu;
+        const double delta = dv<Safe>(delta_factor * nu, omn, Safe ? 0. : rcp<Safe>(omn));
+        const double Nn = c.b_BZ_n * (c.mass + delta * c.b_Dn_sqrte_m2);
+        const double Dn = c.b_D_n * (me + delta * sqrte * c.b_BZ_n);
+        double Phi_n = fm::log(dv<Safe>(Nn, Dn, Safe ? 0. : rcp<Safe>(Dn)), t);
         if (Phi_n < 0.) Phi_n = 0.;
         double Phi_e = 0.;
         if (q < qe_max) {
-            Phi_e = fm::log(c.b_BZe_mass / ((1. + delta * c.b_phie_factor) * (me + delta * sqrte * c.b_BZ_e)), t);
+            const double De = (1. + delta * c.b_phie_factor) * (me + delta * sqrte * c.b_BZ_e);
+            Phi_e = fm::log(dv<Safe>(c.b_BZe_mass, De, Safe ? 0. : rcp<Safe>(De)), t);
             if (Phi_e < 0.) Phi_e = 0.;
         }
-        const double dcs = c.b_dcs_factor * (c.Zd * Phi_n + Phi_e) * (4. / 3. * (1. / nu - 1.) + nu);
-        return (dcs < 0.) ? 0. : dcs * 1E+03 * AVOGADRO_NUMBER / c.A;
+        const double dcs = c.b_dcs_factor * (c.Zd * Phi_n + Phi_e) * (4. / 3. * (rcp<Safe>(nu) - 1.) + nu);
+        return (dcs < 0.) ? 0. : dv<Safe>(dcs * 1E+03 * AVOGADRO_NUMBER, c.A, c.r_A);
     }
 
     // dcs.hh:408-443
     static NOA_DEV_INLINE double ion(double K, double q, const DcsConsts& c, const fm::Tab& t) {
+        if (ordinary(K, q, c)) return ion_impl<false>(K, q, c, t);
+        return ion_impl<true>(K, q, c, t);
+    }
+
+    template <bool Safe>
+    static NOA_DEV_INLINE double ion_impl(double K, double q, const DcsConsts& c, const fm::Tab& t) {
 #pragma clang fp contract(off)
         const double me = ELECTRON_MASS;
         const double P2 = K * (K + c.i_2m);
         const double E = K + c.mass;
-        const double Wmax = 2. * me * P2 / (c.i_m2 + me * (me + 2. * E));
+        const double Dw = c.i_m2 + me * (me + 2. * E);
+        const double Wmax = dv<Safe>(2. * me * P2, Dw, Safe ? 0. : rcp<Safe>(Dw));
         if ((Wmax < X_FRACTION * K) || (q > Wmax)) return 0.;
         if (q <= c.i_Wmin) return 0.;
-        const double a0 = 0.5 / P2;
-        const double a1 = -1. / Wmax;
-        const double a2 = E * E / P2;
-        const double cs = 1.535336E-05 * E * c.Zd / c.A * (a0 + 1. / q * (a1 + a2 / q));
+        const double rP2 = Safe ? 0. : rcp<Safe>(P2);
+        const double a0 = Safe ? 0.5 / P2 : 0.5 * rP2;  // a power-of-two multiple: the same rounding
+        const double a1 = -rcp<Safe>(Wmax);
+        const double a2 = dv<Safe>(E * E, P2, rP2);
+        const double rq = rcp<Safe>(q);
+        const double cs = dv<Safe>(1.535336E-05 * E * c.Zd, c.A, c.r_A) * (a0 + rq * (a1 + dv<Safe>(a2, q, rq)));
         double Delta = 0.;
         if (K >= c.i_Kthr) {
-            const double L1 = fm::log(1. + 2. * q / me, t);
-            Delta = 1.16141E-03 * L1 * (fm::log(4. * E * (E - q) / c.i_m2, t) - L1);
+            const double L1 = fm::log(1. + dv<Safe>(2. * q, me, R_ME), t);
+            Delta = 1.16141E-03 * L1 * (fm::log(dv<Safe>(4. * E * (E - q), c.i_m2, c.i_r_m2), t) - L1);
         }
         return cs * (1. + Delta);
     }
 
     // dcs.hh:151-156 and 168-175: the exits before the integral.
     static NOA_DEV_INLINE bool pair_active(double K, double q, const DcsConsts& c) {
+        if (ordinary(K, q, c)) return pair_active_impl<false>(K, q, c);
+        return pair_active_impl<true>(K, q, c);
+    }
+
+    template <bool Safe>
+    static NOA_DEV_INLINE bool pair_active_impl(double K, double q, const DcsConsts& c) {
 #pragma clang fp contract(off)
         if (q <= 4. * ELECTRON_MASS) return false;
         if (q >= K + c.p_cut) return false;
-        const double gamma = 1. + K / c.mass;
-        const double x0 = 4. * ELECTRON_MASS / q;
-        const double x1 = 6. / (gamma * (gamma - q / c.mass));
-        const double argmin = (x0 + 2. * (1. - x0) * x1) / (1. + (1. - x1) * sqrt(1. - x0));
+        const double gamma = 1. + dv<Safe>(K, c.mass, c.r_mass);
+        const double x0 = dv<Safe>(4. * ELECTRON_MASS, q, Safe ? 0. : rcp<Safe>(q));
+        const double gg = gamma * (gamma - dv<Safe>(q, c.mass, c.r_mass));
+        const double x1 = dv<Safe>(6., gg, Safe ? 0. : rcp<Safe>(gg));
+        const double Na = x0 + 2. * (1. - x0) * x1;
+        double Da = 1. + (1. - x1) * sqrt(1. - x0);
+        if constexpr (!Safe) {
+            // |Da| < 2^-900 (also a signed zero) makes |argmin| > 2^800 or
+            // infinite in the reference: an exit either way, and a positive
+            // 2^-900 gives the same exit without a zero divisor
+            if (fabs(Da) < 0x1p-900) Da = 0x1p-900;
+        }
+        const double argmin = dv<Safe>(Na, Da, Safe ? 0. : rcp<Safe>(Da));
         return !((argmin >= 1.) || (argmin <= 0.));
     }
 
@@ -152,21 +220,38 @@ struct F64N {
 
     // dcs.hh:144-258 for a pair that passed pair_active.
     static NOA_DEV double pair(double K, double q, const DcsConsts& c, const fm::Tab& t) {
-        if (K + c.mass <= FAST_E_MAX) return pair_impl<false>(K, q, c, t);
+        if (c.fast_ok && K + c.mass <= FAST_E_MAX) return pair_impl<false>(K, q, c, t);
         return pair_impl<true>(K, q, c, t);  // also infinite and NaN K
     }
 
-    template <bool Safe>
+    // The same with the set-up quotients as IEEE divisions (Rn = false): the
+    // FP64 recomputation inside the mixed-precision kernel, where the
+    // div_rn path's extra constants raise that kernel's scalar register
+    // spills by more than the divisions cost on ~1% of its pairs.
+    static NOA_DEV double pair_ieee(double K, double q, const DcsConsts& c, const fm::Tab& t) {
+        if (K + c.mass <= FAST_E_MAX) return pair_impl<false, false>(K, q, c, t);
+        return pair_impl<true>(K, q, c, t);
+    }
+
+    template <bool Safe, bool Rn = !Safe>
     static NOA_DEV double pair_impl(double K, double q, const DcsConsts& c, const fm::Tab& t) {
 #pragma clang fp contract(off)
         const double me = ELECTRON_MASS;
-        const double nu = q / (K + c.mass);
-        const double beta = 0.5 * nu * nu / (1. - nu);
+        // the set-up quotients as in pair_active (a pair that passed it has
+        // 4 m_e < q < K + p_cut, p_cut < 0, so K, q > 2e-3 GeV, E - q > 0.76 m
+        // and argmin in (0, 1): every divisor below is positive normal)
+        const double E0 = K + c.mass;
+        const double rq = !Rn ? 0. : rcp<!Rn>(q);
+        const double nu = dv<!Rn>(q, E0, !Rn ? 0. : rcp<!Rn>(E0));
+        const double omn = 1. - nu;
+        const double beta = dv<!Rn>(0.5 * nu * nu, omn, !Rn ? 0. : rcp<!Rn>(omn));
         const double xi_factor = c.p_hr2 * beta;
-        const double gamma = 1. + K / c.mass;
-        const double x0 = 4. * me / q;
-        const double x1 = 6. / (gamma * (gamma - q / c.mass));
-        const double argmin = (x0 + 2. * (1. - x0) * x1) / (1. + (1. - x1) * sqrt(1. - x0));
+        const double gamma = 1. + dv<!Rn>(K, c.mass, c.r_mass);
+        const double x0 = dv<!Rn>(4. * me, q, rq);
+        const double gg = gamma * (gamma - dv<!Rn>(q, c.mass, c.r_mass));
+        const double x1 = dv<!Rn>(6., gg, !Rn ? 0. : rcp<!Rn>(gg));
+        const double Da = 1. + (1. - x1) * sqrt(1. - x0);
+        const double argmin = dv<!Rn>(x0 + 2. * (1. - x0) * x1, Da, !Rn ? 0. : rcp<!Rn>(Da));
         const double tmin = fm::log(argmin, t);
 
         // per-pair coefficients of the node polynomials in rho^2 (explicit
@@ -277,43 +362,62 @@ struct F64N {
         if (gamma <= 35.) {
             zeta = 0.;
         } else {
-            zeta = 0.073 * fm::log(gamma / (1. + c.p_g1 * gamma * c.p_Z13 * c.p_Z13), t) - 0.26;
-            if (zeta <= 0.)
+            const double z1 = 1. + c.p_g1 * gamma * c.p_Z13 * c.p_Z13;
+            zeta = 0.073 * fm::log(dv<!Rn>(gamma, z1, !Rn ? 0. : rcp<!Rn>(z1)), t) - 0.26;
+            if (zeta <= 0.) {
                 zeta = 0.;
-            else
-                zeta /= 0.058 * fm::log(gamma / (1. + c.p_g2 * gamma * c.p_Z13), t) - 0.14;
+            } else {
+                const double z2 = 1. + c.p_g2 * gamma * c.p_Z13;
+                // the divisor 0.058 log(.) - 0.14 can be zero: a plain division
+                zeta /= 0.058 * fm::log(dv<!Rn>(gamma, z2, !Rn ? 0. : rcp<!Rn>(z2)), t) - 0.14;
+            }
         }
         const double E = K + c.mass;
-        const double dcs = c.p_dcs_c * (c.Zd + zeta) * (E - q) * I / (q * E);
-        return (dcs < 0.) ? 0. : dcs * 1E+03 * AVOGADRO_NUMBER * (c.mass + K) / c.A;
+        const double qE = q * E;
+        const double dcs = dv<!Rn>(c.p_dcs_c * (c.Zd + zeta) * (E - q) * I, qE, !Rn ? 0. : rcp<!Rn>(qE));
+        return (dcs < 0.) ? 0. : dv<!Rn>(dcs * 1E+03 * AVOGADRO_NUMBER * (c.mass + K), c.A, c.r_A);
     }
 
     // dcs.hh:357-381: the exits before the integral.
     static NOA_DEV_INLINE bool photo_active(double K, double q, const DcsConsts& c) {
+        if (ordinary(K, q, c)) return photo_active_impl<false>(K, q, c);
+        return photo_active_impl<true>(K, q, c);
+    }
+
+    template <bool Safe>
+    static NOA_DEV_INLINE bool photo_active_impl(double K, double q, const DcsConsts& c) {
 #pragma clang fp contract(off)
         if ((q < 1.) || (q < 2E-03 * K)) return false;
         const double E = K + c.mass;
         if ((q >= (E - c.mass)) || (q <= c.ph_thr_pi)) return false;
-        const double y = q / E;
-        const double Q2min = c.ph_m2 * y * y / (1. - y);
+        // past the exits 1 <= q < K, so y and 1 - y >= m / E are normal
+        const double y = dv<Safe>(q, E, Safe ? 0. : rcp<Safe>(E));
+        const double omy = 1. - y;
+        const double Q2min = dv<Safe>(c.ph_m2 * y * y, omy, Safe ? 0. : rcp<Safe>(omy));
         const double Q2max = 2.0 * PHOTO_M * (q - PHOTO_MPI) - PHOTO_MPI * PHOTO_MPI;
         return !((Q2max < Q2min) | (Q2min < 0.));
     }
 
     // dcs.hh:261-405 for a pair that passed photo_active.
     static NOA_DEV double photo(double K, double q, const DcsConsts& c, const fm::Tab& t) {
-        if (K + c.mass <= FAST_E_MAX) return photo_impl<false>(K, q, c, t);
+        if (c.fast_ok && K + c.mass <= FAST_E_MAX) return photo_impl<false>(K, q, c, t);
         return photo_impl<true>(K, q, c, t);
     }
 
-    template <bool Safe>
+    static NOA_DEV double photo_ieee(double K, double q, const DcsConsts& c, const fm::Tab& t) {
+        if (K + c.mass <= FAST_E_MAX) return photo_impl<false, false>(K, q, c, t);
+        return photo_impl<true>(K, q, c, t);
+    }
+
+    template <bool Safe, bool Rn = !Safe>
     static NOA_DEV double photo_impl(double K, double q, const DcsConsts& c, const fm::Tab& t) {
 #pragma clang fp contract(off)
         const double M = PHOTO_M;
         const double mpi = PHOTO_MPI;
         const double E = K + c.mass;
-        const double y = q / E;
-        const double Q2min = c.ph_m2 * y * y / (1. - y);
+        const double y = dv<!Rn>(q, E, !Rn ? 0. : rcp<!Rn>(E));
+        const double omy = 1. - y;
+        const double Q2min = dv<!Rn>(c.ph_m2 * y * y, omy, !Rn ? 0. : rcp<!Rn>(omy));
         const double Q2max = 2.0 * M * (q - mpi) - mpi * mpi;
         const double pQ2min = fm::log(Q2min, t);
         const double pQ2max = fm::log(Q2max, t);
@@ -413,7 +517,7 @@ struct F64N {
             const double d2 = 2.603096E-35 * F2A * dds * inv_q;
             ds += d2 * Q2 * 1. * GL9_W[j];
         }
-        return (ds < 0.) ? 0. : 0.5 * ds * dpQ2 * 1E+03 * AVOGADRO_NUMBER * (c.mass + K) / c.A;
+        return (ds < 0.) ? 0. : dv<!Rn>(0.5 * ds * dpQ2 * 1E+03 * AVOGADRO_NUMBER * (c.mass + K), c.A, c.r_A);
     }
 
     // The tile runner's interface; FP64 results are final (no recomputation).

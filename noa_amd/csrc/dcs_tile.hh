@@ -163,9 +163,9 @@ __device__ __forceinline__ void run_tile(int64_t base, int64_t end, const double
             const uint32_t tag = s.redo[v];
             const int64_t g = base + (tag & (REDO_PHOTO - 1));
             if (tag & REDO_PHOTO)
-                o.ph[g] = out_t(F64::photo(K[g], q[g], c, t));
+                o.ph[g] = out_t(F64::photo_ieee(K[g], q[g], c, t));
             else
-                o.p[g] = out_t(F64::pair(K[g], q[g], c, t));
+                o.p[g] = out_t(F64::pair_ieee(K[g], q[g], c, t));
         }
     }
     __syncthreads();  // the lists are reused by the next tile of a persistent workgroup

@@ -120,6 +120,30 @@ __device__ __forceinline__ double recip_pos(double x) {
 inline double recip_pos(double x) { return 1.0 / x; }
 #endif
 
+// 1 / x for a nonzero normal x of either sign whose reciprocal is normal:
+// the same steps as recip_pos (the Newton steps are sign-symmetric).
+#if defined(__HIP_DEVICE_COMPILE__)
+__device__ __forceinline__ double recip_nz(double x) { return recip_pos(x); }
+#else
+inline double recip_nz(double x) { return 1.0 / x; }
+#endif
+
+// a / b given rb, the correctly rounded 1 / b: q = a rb refined by one
+// Markstein step, q + (a - b q) rb, with a - b q exact in the FMA. With rb
+// correctly rounded and q within an ulp of a / b this rounds to the same
+// double as the division (Markstein's theorem), for normal b with a normal
+// reciprocal, finite a, and a quotient that is zero or normal. Three VALU
+// instructions against ~10 for the IEEE division sequence, and a shared rb
+// serves every quotient by the same b. The host build divides.
+#if defined(__HIP_DEVICE_COMPILE__)
+__device__ __forceinline__ double div_rn(double a, double b, double rb) {
+    const double q = a * rb;
+    return std::fma(std::fma(-b, q, a), rb, q);
+}
+#else
+inline double div_rn(double a, double b, double) { return a / b; }
+#endif
+
 NOA_HD inline uint64_t as_bits(double x) {
     uint64_t u;
     std::memcpy(&u, &x, sizeof(u));

@@ -16,7 +16,7 @@ pytestmark = pytest.mark.gpu
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 DEV_SO = os.path.join(HERE, "native", "libfm_dev.so")
-FN = {"log": 0, "log_pos": 1, "exp": 2, "exp_fin": 3, "recip_pos": 4}
+FN = {"log": 0, "log_pos": 1, "exp": 2, "exp_fin": 3, "recip_pos": 4, "recip_nz": 5}
 
 
 @pytest.fixture(scope="module")
@@ -26,6 +26,8 @@ def dev():
     lib = ctypes.CDLL(DEV_SO)
     lib.fm_dev_eval.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]
     lib.fm_dev_eval.restype = ctypes.c_int
+    lib.fm_dev_div.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]
+    lib.fm_dev_div.restype = ctypes.c_int
     d = torch.device("cuda", 0)
 
     def call(fn: str, x: np.ndarray) -> np.ndarray:
@@ -35,6 +37,16 @@ def dev():
         rc = lib.fm_dev_eval(FN[fn], xd.data_ptr(), yd.data_ptr(), xd.numel())
         assert rc == 0, f"fm_dev_eval({fn}) returned {rc}"
         return yd.cpu().numpy()
+
+    def div(a: np.ndarray, b: np.ndarray) -> np.ndarray:
+        ad = torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(d)
+        bd = torch.from_numpy(np.ascontiguousarray(b, dtype=np.float64)).to(d)
+        yd = torch.empty_like(ad)
+        torch.cuda.synchronize()
+        rc = lib.fm_dev_div(ad.data_ptr(), bd.data_ptr(), yd.data_ptr(), ad.numel())
+        assert rc == 0, f"fm_dev_div returned {rc}"
+        return yd.cpu().numpy()
+    call.div = div
     return call
 
 
@@ -77,6 +89,35 @@ def test_recip_pos_is_correctly_rounded(dev):
     want = 1.0 / x
     bad = np.flatnonzero(got != want)
     assert bad.size == 0, (bad.size, x[bad[:5]], got[bad[:5]], want[bad[:5]])
+
+
+def test_recip_nz_negative_arguments(dev):
+    """The sign-symmetric reciprocal on negative normal x: identical to 1 / x."""
+    rng = np.random.default_rng(14)
+    x = -(10 ** rng.uniform(-307, 307, 1 << 21))
+    np.testing.assert_array_equal(dev("recip_nz", x), 1.0 / x)
+
+
+def test_div_rn_equals_ieee_division(dev):
+    """div_rn(a, b, recip_nz(b)) against numpy's correctly rounded a / b, bit
+    for bit: random magnitudes and signs over the normal range (quotients
+    kept normal), significands near 1 and 2, exact quotients, and the
+    argument ranges of the closed forms that use it (a / E, q / E, x / A)."""
+    rng = np.random.default_rng(15)
+    n = 1 << 22
+    la = rng.uniform(-150, 150, n)
+    lb = rng.uniform(-150, 150, n)
+    sa = rng.choice([-1.0, 1.0], n)
+    sb = rng.choice([-1.0, 1.0], n)
+    a = np.concatenate([sa * 10 ** la, rng.uniform(1.0, 2.0, 1 << 20), np.arange(1, 1 << 16, dtype=np.float64),
+                        rng.uniform(1e-9, 1e6, 1 << 20), 10 ** rng.uniform(-36, 95, 1 << 20)])
+    b = np.concatenate([sb * 10 ** lb, rng.uniform(1.0, 2.0, 1 << 20),
+                        rng.choice([2.0, 4.0, 5.0, 0.25, 10.0], (1 << 16) - 1),
+                        0.10565839 + 10 ** rng.uniform(-3, 6, 1 << 20), rng.uniform(1.0, 240.0, 1 << 20)])
+    got = dev.div(a, b)
+    want = a / b
+    bad = np.flatnonzero(got != want)
+    assert bad.size == 0, (bad.size, a[bad[:5]], b[bad[:5]], got[bad[:5]], want[bad[:5]])
 
 
 def test_log_exp_match_host_build(dev, host):
