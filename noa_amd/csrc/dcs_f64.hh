@@ -133,12 +133,13 @@ struct F64N {
         const double delta = dv<Safe>(delta_factor * nu, omn, Safe ? 0. : rcp<Safe>(omn));
         const double Nn = c.b_BZ_n * (c.mass + delta * c.b_Dn_sqrte_m2);
         const double Dn = c.b_D_n * (me + delta * sqrte * c.b_BZ_n);
-        double Phi_n = fm::log(dv<Safe>(Nn, Dn, Safe ? 0. : rcp<Safe>(Dn)), t);
+        // fast path: positive normal arguments (N_n, D_n, D_e > 0), so log_pos
+        double Phi_n = nlog<Safe>(dv<Safe>(Nn, Dn, Safe ? 0. : rcp<Safe>(Dn)), t);
         if (Phi_n < 0.) Phi_n = 0.;
         double Phi_e = 0.;
         if (q < qe_max) {
             const double De = (1. + delta * c.b_phie_factor) * (me + delta * sqrte * c.b_BZ_e);
-            Phi_e = fm::log(dv<Safe>(c.b_BZe_mass, De, Safe ? 0. : rcp<Safe>(De)), t);
+            Phi_e = nlog<Safe>(dv<Safe>(c.b_BZe_mass, De, Safe ? 0. : rcp<Safe>(De)), t);
             if (Phi_e < 0.) Phi_e = 0.;
         }
         const double dcs = c.b_dcs_factor * (c.Zd * Phi_n + Phi_e) * (4. / 3. * (rcp<Safe>(nu) - 1.) + nu);
@@ -169,7 +170,10 @@ struct F64N {
         const double cs = dv<Safe>(1.535336E-05 * E * c.Zd, c.A, c.r_A) * (a0 + rq * (a1 + dv<Safe>(a2, q, rq)));
         double Delta = 0.;
         if (K >= c.i_Kthr) {
-            const double L1 = fm::log(1. + dv<Safe>(2. * q, me, R_ME), t);
+            // 1 + 2q/m_e >= 1: log_pos on the fast path. E - q can round to 0
+            // (E = K + m == K for K >> m, q = Wmax = K), where the reference's
+            // log gives -inf: the full log there
+            const double L1 = nlog<Safe>(1. + dv<Safe>(2. * q, me, R_ME), t);
             Delta = 1.16141E-03 * L1 * (fm::log(dv<Safe>(4. * E * (E - q), c.i_m2, c.i_r_m2), t) - L1);
         }
         return cs * (1. + Delta);
@@ -252,7 +256,7 @@ struct F64N {
         const double x1 = dv<!Rn>(6., gg, !Rn ? 0. : rcp<!Rn>(gg));
         const double Da = 1. + (1. - x1) * sqrt(1. - x0);
         const double argmin = dv<!Rn>(x0 + 2. * (1. - x0) * x1, Da, !Rn ? 0. : rcp<!Rn>(Da));
-        const double tmin = fm::log(argmin, t);
+        const double tmin = nlog<Safe>(argmin, t);  // argmin in (0, 1), >= x0 / 2 > 1e-33
 
         // per-pair coefficients of the node polynomials in rho^2 (explicit
         // FMAs: with contraction off the compiler keeps every product and
@@ -363,13 +367,13 @@ struct F64N {
             zeta = 0.;
         } else {
             const double z1 = 1. + c.p_g1 * gamma * c.p_Z13 * c.p_Z13;
-            zeta = 0.073 * fm::log(dv<!Rn>(gamma, z1, !Rn ? 0. : rcp<!Rn>(z1)), t) - 0.26;
+            zeta = 0.073 * nlog<Safe>(dv<!Rn>(gamma, z1, !Rn ? 0. : rcp<!Rn>(z1)), t) - 0.26;
             if (zeta <= 0.) {
                 zeta = 0.;
             } else {
                 const double z2 = 1. + c.p_g2 * gamma * c.p_Z13;
                 // the divisor 0.058 log(.) - 0.14 can be zero: a plain division
-                zeta /= 0.058 * fm::log(dv<!Rn>(gamma, z2, !Rn ? 0. : rcp<!Rn>(z2)), t) - 0.14;
+                zeta /= 0.058 * nlog<Safe>(dv<!Rn>(gamma, z2, !Rn ? 0. : rcp<!Rn>(z2)), t) - 0.14;
             }
         }
         const double E = K + c.mass;
@@ -419,8 +423,9 @@ struct F64N {
         const double omy = 1. - y;
         const double Q2min = dv<!Rn>(c.ph_m2 * y * y, omy, !Rn ? 0. : rcp<!Rn>(omy));
         const double Q2max = 2.0 * M * (q - mpi) - mpi * mpi;
-        const double pQ2min = fm::log(Q2min, t);
-        const double pQ2max = fm::log(Q2max, t);
+        // 0 < Q2min < Q2max (photo_active), Q2min >= m^2 y^2 > 1e-72 below FAST_E_MAX
+        const double pQ2min = nlog<Safe>(Q2min, t);
+        const double pQ2max = nlog<Safe>(Q2max, t);
         const double dpQ2 = pQ2max - pQ2min;
         const double pQ2c = 0.5 * (pQ2max + pQ2min);
         const double hd = 0.5 * dpQ2;
