@@ -134,15 +134,13 @@ inline double recip_nz(double x) { return 1.0 / x; }
 // double as the division (Markstein's theorem), for normal b with a normal
 // reciprocal, finite a, and a quotient that is zero or normal. Three VALU
 // instructions against ~10 for the IEEE division sequence, and a shared rb
-// serves every quotient by the same b. The host build divides.
-#if defined(__HIP_DEVICE_COMPILE__)
-__device__ __forceinline__ double div_rn(double a, double b, double rb) {
+// serves every quotient by the same b. The host build runs the same steps
+// (its reciprocals are IEEE divisions, the values the device's equal), so
+// the CPU tests of the physics exercise this path.
+NOA_HD inline double div_rn(double a, double b, double rb) {
     const double q = a * rb;
     return std::fma(std::fma(-b, q, a), rb, q);
 }
-#else
-inline double div_rn(double a, double b, double) { return a / b; }
-#endif
 
 NOA_HD inline uint64_t as_bits(double x) {
     uint64_t u;

@@ -27,7 +27,15 @@ def fm(tmp_path_factory):
         y = np.empty_like(x)
         f(x.ctypes.data_as(D), y.ctypes.data_as(D), x.size)
         return y
-    return {"log": lambda x: call(lib.fm_log_batch, x), "exp": lambda x: call(lib.fm_exp_batch, x)}
+    lib.fm_div_rn_batch.argtypes = [D, D, D, ctypes.c_long]
+
+    def div(a, b):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        y = np.empty_like(a)
+        lib.fm_div_rn_batch(a.ctypes.data_as(D), b.ctypes.data_as(D), y.ctypes.data_as(D), a.size)
+        return y
+    return {"log": lambda x: call(lib.fm_log_batch, x), "exp": lambda x: call(lib.fm_exp_batch, x), "div": div}
 
 
 def ulp_err(got: np.ndarray, exact: list) -> np.ndarray:
@@ -107,3 +115,20 @@ def test_exp_extreme_arguments(fm):
     assert max(float(abs(Decimal(float(g)) - e) / tiny) for g, e in zip(got, exact)) <= 1.0
     sp = fm["exp"](np.array([709.8, 750.0, 1e300, np.inf, -745.2, -750.0, -1e300, -np.inf, np.nan]))
     assert np.all(sp[:4] == np.inf) and np.all(sp[4:8] == 0.0) and np.isnan(sp[8])
+
+
+def test_div_rn_is_the_division(fm):
+    """The Markstein quotient q + (a - b q) / b with a correctly rounded 1 / b
+    (what the device's Newton reciprocal gives, pinned by the GPU tests)
+    rounds as IEEE division: random signs and magnitudes over the normal
+    range, significands near 1 and 2, exact quotients."""
+    rng = np.random.default_rng(21)
+    n = 1 << 20
+    a = np.concatenate([rng.choice([-1.0, 1.0], n) * 10 ** rng.uniform(-150, 150, n), rng.uniform(1, 2, n),
+                        np.arange(1, 4097, dtype=np.float64)])
+    b = np.concatenate([rng.choice([-1.0, 1.0], n) * 10 ** rng.uniform(-150, 150, n), rng.uniform(1, 2, n),
+                        rng.choice([2.0, 3.0, 5.0, 0.25, 10.0], 4096)])
+    got = fm["div"](a, b)
+    want = a / b
+    bad = np.flatnonzero(got != want)
+    assert bad.size == 0, (bad.size, a[bad[:4]], b[bad[:4]])
