@@ -10,7 +10,8 @@
 //                    (q rho21 + c_L x_e)^2)), with no square root; c_L / rho21
 //                    is folded into the log arguments; (rho21 - beta) / (1 +
 //                    xi) is shared by B_e and B_mu; / r^2 is a product with
-//                    1 / r^2; each node takes 1 / xi as the reference does and
+//                    1 / r^2; each node takes a correctly rounded 1 / xi (the
+//                    Newton reciprocal, the reference's quotient) and
 //                    two reciprocals of products (the two Y denominators with
 //                    1 + xi; the two screening denominators) where the
 //                    reference divides five more times;
@@ -26,9 +27,10 @@
 // energies up to 1e30 GeV: 1 + 1/xi, 3 + xi, the screening arguments, Q^2's
 // exponent, (Q^2 + Q_0^2) / Lambda^2 >= 8, 1 - x >= m_pi / q, x_P, x_R and
 // the ALLM exponents. NaN inputs still give NaN results through the
-// arithmetic around them. Above 1e30 GeV, and for infinite or NaN K, the
-// integrals run with the full log / exp (the same code, templated), so the
-// reference's overflow results are kept.
+// arithmetic around them. Above 1e30 GeV, for infinite or NaN K, and for an
+// element outside the fast_ok range, the integrals run with the full log /
+// exp and IEEE divisions (the same code, templated), so the reference's
+// overflow results are kept.
 // The quotients of the closed forms, the exits and the integrals' set-up
 // (a / b in the reference) are fm::div_rn(a, b, 1 / b): a correctly rounded
 // reciprocal refined by one Markstein step, which rounds as the division
