@@ -73,6 +73,23 @@ def test_f32mixed_error(el, device):
         assert r["max_rel"] <= F32_TOL, r
 
 
+@pytest.mark.parametrize("el,mass", [(AtomicElement(A=0.9, I=19.2e-9, Z=1), MUON_MASS),
+                                     (STANDARD_ROCK, 2e6), (STANDARD_ROCK, 5e-7)],
+                         ids=["A0.9", "mass2e6", "mass5e-7"])
+def test_f64_parity_outside_fast_range(el, mass, device):
+    """Elements and masses outside the range where the kernels' closed forms
+    divide through correctly rounded reciprocals (1 <= A <= 1e4 g/mol, 1e-6 <=
+    m <= 1e6 GeV) take the IEEE-division instantiation: still the oracle's
+    values."""
+    K, q = workload.pairs(1 << 14, seed=7)
+    exp = oracle.dcs_map_all(K, q, el, mass, threads=0)
+    got = _gpu_all(K, q, el, mass, device)
+    for p in range(4):
+        r = compare(got[p], exp[p])
+        print(f"A={el.A} m={mass} process {p}: {r}", flush=True)
+        assert r["zero_mismatch"] == 0 and r["max_rel"] <= F64_TOL, (p, r)
+
+
 def test_f64_special_inputs(device):
     """NaN, infinite, zero, negative, subnormal and extreme (up to 1e300 GeV)
     K and q: the FP64 kernel gives the oracle's values, NaNs and infinities

@@ -55,6 +55,20 @@ def test_parity_48_elements_two_masses(host_eval):
     assert (worst <= 1e-10).all(), worst
 
 
+@pytest.mark.parametrize("el,mass", [(AtomicElement(A=0.9, I=19.2e-9, Z=1), MUON_MASS),
+                                     (STANDARD_ROCK, 2e6), (STANDARD_ROCK, 5e-7)],
+                         ids=["A0.9", "mass2e6", "mass5e-7"])
+def test_parity_outside_fast_range(host_eval, el, mass):
+    """Outside 1 <= A <= 1e4 g/mol and 1e-6 <= m <= 1e6 GeV the closed forms
+    and set-ups run with IEEE divisions: the oracle's values still."""
+    K, q = workload.pairs(1 << 12, seed=7)
+    exp = oracle.dcs_map_all(K, q, el, mass, threads=0)
+    got = host_eval(K, q, el, mass)
+    for p in range(4):
+        r = compare(got[p], exp[p])
+        assert r["zero_mismatch"] == 0 and r["max_rel"] <= 1e-10, (p, r)
+
+
 def test_tau_near_xi_1e3(host_eval):
     """B_e cancels ~3000-fold near xi = 1e3: 1/xi must round as the
     reference's division does (a reciprocal of xi (1 + xi) gave 1.1e-10 at
