@@ -95,6 +95,41 @@ def test_batch_mixed_unroll_matches(small_batch, device):
         p.close()
 
 
+def test_autotune_cache_round_trip(small_batch, device, tmp_path):
+    """A full autotune appends one line (key, tile, workgroups per CU,
+    unroll) to the cache file; a second plan on the same shape reads it back,
+    makes the same choice without tuning and gives identical results. A
+    line with values out of range is ignored (the plan tunes instead)."""
+    elements, offsets, Kh, qh, exp = small_batch
+    K = torch.from_numpy(Kh).to(device)
+    q = torch.from_numpy(qh).to(device)
+    cache = tmp_path / "tune.txt"
+    first = BatchPlan(elements, offsets, K, q, MUON_MASS, tune_cache=str(cache))
+    first.launch()
+    torch.cuda.synchronize()
+    info = first.info()
+    lines = cache.read_text().splitlines()
+    assert len(lines) == 1, lines
+    key, tile, wpc, unroll = lines[0].split()
+    assert (int(tile), int(wpc), int(unroll)) == (info["tile"], info["waves_per_cu"], info["unroll"]), (lines, info)
+    ref = [o.clone() for o in first.outputs]
+    first.close()
+    second = BatchPlan(elements, offsets, K, q, MUON_MASS, tune_cache=str(cache))
+    second.launch()
+    torch.cuda.synchronize()
+    assert second.info() == info
+    assert len(cache.read_text().splitlines()) == 1  # read, not tuned and appended again
+    for a, b in zip(second.outputs, ref):
+        assert torch.equal(a, b)
+    second.close()
+    # a later line with the same key but an invalid tile is skipped: the plan tunes and appends
+    cache.write_text(f"{key} 999 0 7\n")
+    third = BatchPlan(elements, offsets, K, q, MUON_MASS, tune_cache=str(cache))
+    assert third.info()["tile"] in (1024, 2048, 4096)
+    assert len(cache.read_text().splitlines()) == 2
+    third.close()
+
+
 def test_device_batch_generator_matches_host(device):
     elements, offsets = workload.element_batch(6, seed=2, log2_min=8, log2_max=10)
     Kh, qh = workload.batch_pairs(offsets, seed=2)
