@@ -320,6 +320,21 @@ def main() -> None:
             forms[f"batched48_{prec}"] = {"evals_per_s": nb * args.steps / wb, "kernel_ms": kb,
                                           "n_pairs": nb, **plan.info()}
             plan.close()
+        # the same batch as one single-element launch per element, the way the
+        # reference's adaptors take one element per call
+        outs_b = tuple(torch.empty_like(Kb) for _ in range(4))
+
+        def per_element():
+            for e, el in enumerate(elements):
+                a, b = int(offsets[e]), int(offsets[e + 1])
+                if b > a:
+                    dcs.eval_f64(Kb[a:b], qb[a:b], el, MUON_MASS, *(o[a:b] for o in outs_b))
+        wl, kl = timed_launches(per_element, args.steps, args.warmup, stream)
+        forms["per_element_loop48_f64"] = {"evals_per_s": nb * args.steps / wl, "ms_per_batch": kl,
+                                           "n_pairs": nb, "launches": len(elements),
+                                           "config": "48 dcs.eval_f64 calls from Python per batch (host "
+                                                     "dispatch between the launches included)"}
+        del outs_b
         result["forms"] = forms
 
     if args.calibrate_copy:
