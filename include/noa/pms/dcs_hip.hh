@@ -15,10 +15,14 @@
 // src/noa/pms/physics.hh:29-76 that these calls take are declared here too.
 //
 // Differences from the reference, on purpose: the tensors must be
-// contiguous float64 tensors on a ROCm device with equal numel (checked with
-// TORCH_CHECK; the reference checks only the dtype), work is enqueued on the
-// current HIP stream of that device, pvmap/pmap run the same GPU kernels as
-// vmap/map, and dcs::hip adds fused four-process and mixed-precision forms.
+// contiguous float64 tensors with equal numel on one device (checked with
+// TORCH_CHECK; the reference checks only the dtype). vmap/map/pvmap/pmap
+// dispatch on that device: GPU tensors go to the gfx950 kernels, enqueued on
+// the current HIP stream of the device (pvmap/pmap run the same kernels as
+// vmap/map); CPU tensors go to the same FP64 physics compiled for the host,
+// serially for vmap/map and with an OpenMP team for pvmap/pmap, as the
+// reference's CPU adaptors. The scalar functors evaluate one pair on the CPU.
+// dcs::hip adds fused four-process and mixed-precision forms (GPU only).
 #pragma once
 
 #include <torch/types.h>
@@ -72,9 +76,10 @@ inline constexpr Process ionisation{3};
 
 namespace detail {
 
-inline void check_input(const torch::Tensor& t, const char* name) {
+inline void check_input(const torch::Tensor& t, const char* name, bool allow_cpu = false) {
     TORCH_CHECK(t.scalar_type() == torch::kFloat64, name, " must be float64, got ", t.scalar_type());
-    TORCH_CHECK(t.is_cuda(), name, " must be on a GPU device (there is no CPU path), got ", t.device());
+    TORCH_CHECK(t.is_cuda() || (allow_cpu && t.is_cpu()), name, " must be on a GPU device",
+                allow_cpu ? "" : " (this form has no CPU path)", ", got ", t.device());
     TORCH_CHECK(t.is_contiguous(), name, " must be contiguous");
 }
 
@@ -85,9 +90,9 @@ inline void check_output(const torch::Tensor& r, const torch::Tensor& K, torch::
     TORCH_CHECK(r.numel() == K.numel(), "result has ", r.numel(), " elements, kinetic_energies ", K.numel());
 }
 
-inline void check_pair(const torch::Tensor& K, const torch::Tensor& q) {
-    check_input(K, "kinetic_energies");
-    check_input(q, "recoil_energies");
+inline void check_pair(const torch::Tensor& K, const torch::Tensor& q, bool allow_cpu = false) {
+    check_input(K, "kinetic_energies", allow_cpu);
+    check_input(q, "recoil_energies", allow_cpu);
     TORCH_CHECK(q.numel() == K.numel(), "recoil_energies has ", q.numel(), " elements, kinetic_energies ",
                 K.numel());
     TORCH_CHECK(q.device() == K.device(), "kinetic_energies and recoil_energies are on different devices");
@@ -101,10 +106,12 @@ inline hipStream_t stream_of(const torch::Tensor& t) {
     return c10::hip::getCurrentHIPStream(t.device().index()).stream();
 }
 
-// One fused FP64 launch; a null tensor pointer skips that process.
+// FP64 evaluation; a null tensor pointer skips that process. GPU tensors:
+// one fused launch. CPU tensors: the host build, serial for threads == 1,
+// the OpenMP default team for threads == 0.
 inline void eval_f64(const torch::Tensor& K, const torch::Tensor& q, const AtomicElement& el, ParticleMass mass,
-                     const std::array<const torch::Tensor*, 4>& outs) {
-    check_pair(K, q);
+                     const std::array<const torch::Tensor*, 4>& outs, int threads = 0) {
+    check_pair(K, q, /*allow_cpu=*/true);
     double* p[4] = {nullptr, nullptr, nullptr, nullptr};
     for (int i = 0; i < 4; i++) {
         if (outs[i]) {
@@ -112,9 +119,13 @@ inline void eval_f64(const torch::Tensor& K, const torch::Tensor& q, const Atomi
             p[i] = outs[i]->data_ptr<double>();
         }
     }
-    const c10::DeviceGuard guard(K.device());
     const noa_dcs_element e{el.A, el.I, el.Z};
     const noa_dcs_outputs_f64 o{p[0], p[1], p[2], p[3]};
+    if (K.is_cpu()) {
+        raise(noa_dcs_eval_f64_host(K.data_ptr<double>(), q.data_ptr<double>(), K.numel(), &e, mass, &o, threads));
+        return;
+    }
+    const c10::DeviceGuard guard(K.device());
     raise(noa_dcs_eval_f64(K.data_ptr<double>(), q.data_ptr<double>(), K.numel(), &e, mass, &o, stream_of(K)));
 }
 
@@ -137,36 +148,51 @@ inline void eval_f32mixed(const torch::Tensor& K, const torch::Tensor& q, const 
 
 }  // namespace detail
 
-// dcs.hh:35-48
-inline auto vmap(const Process& dcs_func) {
-    return [dcs_func](const Calculation& result, const Energies& kinetic_energies, const Energies& recoil_energies,
-                      const AtomicElement& element, const ParticleMass& mass) {
+namespace detail {
+
+inline auto vmap_threads(const Process& dcs_func, int threads) {
+    return [dcs_func, threads](const Calculation& result, const Energies& kinetic_energies,
+                               const Energies& recoil_energies, const AtomicElement& element,
+                               const ParticleMass& mass) {
         std::array<const torch::Tensor*, 4> outs{nullptr, nullptr, nullptr, nullptr};
         outs[dcs_func.id] = &result;
-        detail::eval_f64(kinetic_energies, recoil_energies, element, mass, outs);
+        eval_f64(kinetic_energies, recoil_energies, element, mass, outs, threads);
     };
 }
 
-// dcs.hh:50-60
-inline auto map(const Process& dcs_func) {
-    return [dcs_func](const Energies& kinetic_energies, const Energies& recoil_energies,
-                      const AtomicElement& element, const ParticleMass& mass) {
+inline auto map_threads(const Process& dcs_func, int threads) {
+    return [dcs_func, threads](const Energies& kinetic_energies, const Energies& recoil_energies,
+                               const AtomicElement& element, const ParticleMass& mass) {
         const auto result = torch::zeros_like(kinetic_energies);
-        vmap(dcs_func)(result, kinetic_energies, recoil_energies, element, mass);
+        vmap_threads(dcs_func, threads)(result, kinetic_energies, recoil_energies, element, mass);
         return result;
     };
 }
 
-// dcs.hh:62-87: the OpenMP forms; on the GPU the same kernels.
-inline auto pvmap(const Process& dcs_func) { return vmap(dcs_func); }
-inline auto pmap(const Process& dcs_func) { return map(dcs_func); }
+}  // namespace detail
 
+// dcs.hh:35-48 (serial on CPU tensors)
+inline auto vmap(const Process& dcs_func) { return detail::vmap_threads(dcs_func, 1); }
+
+// dcs.hh:50-60 (serial on CPU tensors)
+inline auto map(const Process& dcs_func) { return detail::map_threads(dcs_func, 1); }
+
+// dcs.hh:62-87: the OpenMP forms on CPU tensors (OMP_NUM_THREADS); on GPU
+// tensors the same kernels as vmap/map.
+inline auto pvmap(const Process& dcs_func) { return detail::vmap_threads(dcs_func, 0); }
+inline auto pmap(const Process& dcs_func) { return detail::map_threads(dcs_func, 0); }
+
+// The scalar functor: one pair on the CPU (the host build of the kernels'
+// FP64 physics), as the reference's functors run on the host.
 inline Scalar Process::operator()(const Scalar& K, const Scalar& q, const AtomicElement& el,
                                   const ParticleMass& mass) const {
-    const auto opts = torch::TensorOptions().dtype(torch::kFloat64).device(torch::kCUDA);
-    const auto k = torch::full({1}, K, opts);
-    const auto qq = torch::full({1}, q, opts);
-    return map(*this)(k, qq, el, mass).item<Scalar>();
+    double r = 0.;
+    double* p[4] = {nullptr, nullptr, nullptr, nullptr};
+    p[id] = &r;
+    const noa_dcs_element e{el.A, el.I, el.Z};
+    const noa_dcs_outputs_f64 o{p[0], p[1], p[2], p[3]};
+    detail::raise(noa_dcs_eval_f64_host(&K, &q, 1, &e, mass, &o, 1));
+    return r;
 }
 
 // ---- recoil integrals (dcs.hh:89-130, 444-497, 955-1001) ----
@@ -308,10 +334,12 @@ namespace hip {
     inline void vmap_##name(const Calculation& result, const Energies& kinetic_energies,                   \
                             const Energies& recoil_energies, const AtomicElement& element,                 \
                             const ParticleMass& mass) {                                                    \
+        detail::check_pair(kinetic_energies, recoil_energies);                                             \
         vmap(name)(result, kinetic_energies, recoil_energies, element, mass);                              \
     }                                                                                                      \
     inline Calculation map_##name(const Energies& kinetic_energies, const Energies& recoil_energies,      \
                                   const AtomicElement& element, const ParticleMass& mass) {                \
+        detail::check_pair(kinetic_energies, recoil_energies);                                             \
         return map(name)(kinetic_energies, recoil_energies, element, mass);                                \
     }
 NOA_DCS_HIP_PROCESS(bremsstrahlung)
@@ -323,6 +351,7 @@ NOA_DCS_HIP_PROCESS(ionisation)
 // All four processes in one launch (K and q read once).
 inline void vmap_all(const std::array<Calculation, 4>& results, const Energies& kinetic_energies,
                      const Energies& recoil_energies, const AtomicElement& element, const ParticleMass& mass) {
+    detail::check_pair(kinetic_energies, recoil_energies);
     detail::eval_f64(kinetic_energies, recoil_energies, element, mass,
                      {&results[0], &results[1], &results[2], &results[3]});
 }

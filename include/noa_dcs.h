@@ -78,6 +78,18 @@ int noa_dcs_eval_f64(const double* K, const double* q, int64_t n, const noa_dcs_
                      double mass, const noa_dcs_outputs_f64* out, hipStream_t stream);
 
 /*
+ * The same FP64 evaluation on the CPU, for host arrays (K, q and the outputs
+ * are host pointers; no stream, returns when done). Replaces the reference's
+ * CPU adaptors dcs::vmap / dcs::map (serial, src/noa/pms/dcs.hh:35-60) and
+ * dcs::pvmap / dcs::pmap (OpenMP, dcs.hh:62-87; loops of
+ * src/noa/utils/common.hh:146-183): n_threads = 1 runs serially, 0 uses the
+ * OpenMP default team (OMP_NUM_THREADS), k > 1 uses k threads. The physics is
+ * the GPU kernels' FP64 code compiled for the host.
+ */
+int noa_dcs_eval_f64_host(const double* K, const double* q, int64_t n, const noa_dcs_element* element,
+                          double mass, const noa_dcs_outputs_f64* out, int32_t n_threads);
+
+/*
  * Mixed-precision evaluation: FP32 arithmetic with FP64 kept for the
  * kinematic differences (E - q, K - q, 1 - nu), the pair-production
  * integration bound and the photonuclear Q^2 range. FP64 inputs, FP32

@@ -68,6 +68,8 @@ SIGNATURES = {
     "noa_dcs_last_error": (ctypes.c_char_p, []),
     "noa_dcs_version": (ctypes.c_char_p, []),
     "noa_dcs_eval_f64": (ctypes.c_int, [_P, _P, _I64, ctypes.POINTER(Element), _D, ctypes.POINTER(OutputsF64), _P]),
+    "noa_dcs_eval_f64_host": (ctypes.c_int, [_P, _P, _I64, ctypes.POINTER(Element), _D,
+                                             ctypes.POINTER(OutputsF64), _I32]),
     "noa_dcs_eval_f32mixed": (ctypes.c_int, [_P, _P, _I64, ctypes.POINTER(Element), _D,
                                              ctypes.POINTER(OutputsF32), _P]),
     "noa_dcs_batch_create": (ctypes.c_int, [ctypes.POINTER(Element), ctypes.POINTER(_I64), _I32, _D, _P, _P,

@@ -21,6 +21,10 @@ BUILD_DIR = os.path.join(HERE, "_build")
 
 HIP_SOURCES = ["dcs_eval.hip", "dcs_batch.hip", "dcs_recoil.hip", "dcs_coulomb.hip", "surrogate_train.hip", "surrogate_infer.hip"]
 
+# host-compiled sources of the library (the CPU-tensor path), built with g++
+HOST_SOURCES = ["dcs_host.cc"]
+HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-fopenmp", "-ffp-contract=off", "-Wno-unknown-pragmas"]
+
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [
     "--offload-arch=gfx950",
@@ -52,10 +56,22 @@ def build_library(verbose: bool = False) -> str:
             print(" ".join(cmd), flush=True)
         _run(cmd)
 
-    with ThreadPoolExecutor(max_workers=len(srcs)) as ex:
-        list(ex.map(compile_one, range(len(srcs))))
+    hsrcs = [s for s in HOST_SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    hobjs = [os.path.join(BUILD_DIR, os.path.splitext(s)[0] + ".host.o") for s in hsrcs]
+
+    def compile_host(i: int) -> None:
+        cmd = ["g++", *HOST_FLAGS, "-c", os.path.join(CSRC, hsrcs[i]), "-o", hobjs[i]]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        _run(cmd)
+
+    with ThreadPoolExecutor(max_workers=len(srcs) + len(hsrcs)) as ex:
+        futs = [ex.submit(compile_one, i) for i in range(len(srcs))]
+        futs += [ex.submit(compile_host, i) for i in range(len(hsrcs))]
+        for f in futs:
+            f.result()
     tmp = LIB + ".tmp"
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs])
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs, *hobjs, "-lgomp"])
     os.replace(tmp, LIB)
     return LIB
 

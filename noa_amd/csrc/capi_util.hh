@@ -1,48 +1,15 @@
-// Error reporting and argument checks shared by the C entry points.
+// Error reporting and argument checks shared by the C entry points of the
+// HIP sources.
 #pragma once
 
 #include <hip/hip_runtime.h>
 
-#include <cmath>
-#include <cstdarg>
-#include <cstdio>
-#include <string>
-
-#include "../../include/noa_dcs.h"
+#include "capi_error.hh"
 
 namespace noa_hip {
 
-inline std::string& last_error_slot() {
-    static thread_local std::string msg;
-    return msg;
-}
-
-inline int fail(int code, const char* fmt, ...) {
-    char buf[512];
-    va_list ap;
-    va_start(ap, fmt);
-    vsnprintf(buf, sizeof(buf), fmt, ap);
-    va_end(ap);
-    last_error_slot() = buf;
-    return code;
-}
-
-inline int ok() {
-    last_error_slot().clear();
-    return NOA_DCS_OK;
-}
-
 inline int check_hip(hipError_t e, const char* what) {
     if (e != hipSuccess) return fail(NOA_DCS_ELAUNCH, "%s: %s", what, hipGetErrorString(e));
-    return NOA_DCS_OK;
-}
-
-inline int check_element(const noa_dcs_element* el, double mass) {
-    if (!el) return fail(NOA_DCS_EINVAL, "element is NULL");
-    if (!(el->A > 0.0) || !std::isfinite(el->A)) return fail(NOA_DCS_EINVAL, "element A must be > 0 (got %g)", el->A);
-    if (!(el->I >= 0.0) || !std::isfinite(el->I)) return fail(NOA_DCS_EINVAL, "element I must be >= 0 (got %g)", el->I);
-    if (el->Z < 1) return fail(NOA_DCS_EINVAL, "element Z must be >= 1 (got %d)", el->Z);
-    if (!(mass > 0.0) || !std::isfinite(mass)) return fail(NOA_DCS_EINVAL, "mass must be > 0 (got %g)", mass);
     return NOA_DCS_OK;
 }
 

@@ -111,12 +111,12 @@ struct F32MN {
         const float BZn = float(c.b_BZ_n), BZe = float(c.b_BZ_e), Dn = float(c.b_D_n);
         float Phi_n = __logf(fdiv(BZn * (float(c.mass) + delta * float(c.b_Dn_sqrte_m2)),
                                   Dn * (me + delta * sqrte * BZn)));
-        Phi_n = fmaxf(Phi_n, 0.f);
+        if (Phi_n < 0.f) Phi_n = 0.f;  // a NaN stays NaN, as in physics.hh:140
         float Phi_e = 0.f;
         // q < E / (1 + m^2 / (2 me E)) without the two divisions
         if (q * (ELECTRON_MASS * E + c.b_half_m2) < ELECTRON_MASS * E * E) {
             Phi_e = __logf(fdiv(float(c.b_BZe_mass), (1.f + delta * float(c.b_phie_factor)) * (me + delta * sqrte * BZe)));
-            Phi_e = fmaxf(Phi_e, 0.f);
+            if (Phi_e < 0.f) Phi_e = 0.f;
         }
         const float Zf = float(c.Zd);
         float S = Zf * Phi_n + Phi_e;
@@ -240,8 +240,11 @@ struct F32MN {
                 zeta = fdiv(zeta, 0.058f * __logf(fdiv(gamma, 1.f + float(c.p_g2) * gamma * Z13)) - 0.14f);
         }
         const float dcs = float(c.f_pair_c) * (float(c.Zd) + zeta) * fdiv(float(E - q) * I, qf);
-        redo = err * fabsf(tmin) > PAIR_REDO_TOL * I;
-        return fmaxf(dcs, 0.f);
+        // A NaN integral or result (NaN or infinite inputs that pass the
+        // exits) fails the estimate and is recomputed by the FP64 physics,
+        // which carries the NaN as dcs.hh:257 does.
+        redo = !(err * fabsf(tmin) <= PAIR_REDO_TOL * I) || dcs != dcs;
+        return (dcs < 0.f) ? 0.f : dcs;
     }
 
     static __device__ __forceinline__ bool photo_active(double K, double q, const DcsConsts& c) {
@@ -325,7 +328,8 @@ struct F32MN {
             ds = fmaf(GL9_WF[j], F2A * ddsQ2, ds);
         }
         const float r = float(c.f_photo_c) * float(dpQ2_d) * fdiv(Ef, qf) * ds;
-        return fmaxf(r, 0.f);
+        redo = r != r;  // NaN: recomputed in FP64, which carries it as dcs.hh:404 does
+        return (r < 0.f) ? 0.f : r;
     }
 };
 

@@ -50,7 +50,7 @@
 #include <hip/hip_runtime.h>
 #define NOA_DEV __device__
 #define NOA_DEV_INLINE __device__ __forceinline__
-#else  // host build of the same arithmetic, for the CPU tests
+#else  // host build of the same arithmetic: the CPU-tensor path (dcs_host.cc) and the CPU tests
 #include <cmath>
 #define NOA_DEV
 #define NOA_DEV_INLINE inline

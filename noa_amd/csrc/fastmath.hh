@@ -233,13 +233,16 @@ NOA_HD inline double exp(double x, const Tab& t) {
     return std::ldexp(y, m);
 }
 
-// exp of a finite |x| < 1e12 (or NaN, which propagates), without the NaN
-// test and the clamp; overflow and underflow still come out of ldexp as inf,
-// 0 or subnormal. An infinite x gives NaN. k = rint(128 x / ln2) is read
-// from the low mantissa bits of 128 x / ln2 + 1.5 * 2^52, formed by one FMA
-// (no float-to-int conversion, so no out-of-range case; k may differ from
-// the rounded product's at a tie, which leaves |r| within ln2 / 256 + 1 ulp,
-// far inside the polynomial's range).
+// exp of a finite |x| < 2^31 ln2 / 128 (about 1.16e7), or NaN, which
+// propagates, without the NaN test and the clamp; overflow and underflow
+// still come out of ldexp as inf, 0 or subnormal. An infinite x gives NaN.
+// k = rint(128 x / ln2) is read as a signed 32-bit integer from the low
+// mantissa bits of 128 x / ln2 + 1.5 * 2^52, formed by one FMA (no
+// float-to-int conversion; k may differ from the rounded product's at a tie,
+// which leaves |r| within ln2 / 256 + 1 ulp, far inside the polynomial's
+// range). Beyond the stated range k wraps and the result is wrong; the
+// callers' arguments are node offsets and logarithms of physical ratios,
+// |x| < 1e3.
 NOA_HD inline double exp_fin(double x, const Tab& t) {
 #pragma clang fp contract(off)
     const double shifted = std::fma(x, INV_LN2_N, 0x1.8p52);

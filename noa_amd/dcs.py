@@ -9,12 +9,19 @@ GPU entry points (src/noa/pms/dcs.hh:1004-1019, dcs.cuh:30-51):
     dcs.hip.vmap_bremsstrahlung(result, K, q, STANDARD_ROCK, MUON_MASS)
     b, p, ph, i = dcs.hip.map_all(K, q, STANDARD_ROCK, MUON_MASS)
 
-Tensors must be float64, contiguous, on a ROCm (torch "cuda") device, with
-K, q and result of equal numel. Unlike the reference (which checks only the
-dtype, through data_ptr<double>()), size, device and contiguity are checked
-and raise ValueError. Work is enqueued on torch's current stream. There is
-no CPU path: `pvmap`/`pmap` (the reference's OpenMP forms) run the same GPU
-kernels as `vmap`/`map`.
+Tensors must be float64 and contiguous, K, q and result of equal numel and
+on one device. Unlike the reference (which checks only the dtype, through
+data_ptr<double>()), size, device and contiguity are checked and raise
+ValueError. The adaptors dispatch on the tensors' device:
+  - on a ROCm (torch "cuda") device, the gfx950 kernels, enqueued on torch's
+    current stream; `pvmap`/`pmap` run the same kernels as `vmap`/`map`;
+  - on the CPU, the same FP64 physics compiled for the host
+    (noa_dcs_eval_f64_host): `vmap`/`map` serially as the reference's
+    dcs::vmap/map, `pvmap`/`pmap` with an OpenMP team as its dcs::pvmap/pmap.
+The scalar functor form (`dcs.pair_production(K, q, el, m)`) evaluates one
+pair on the CPU, as the reference's functors do. The `hip` namespace (the
+reference's dcs::cuda entry points and the fused and mixed-precision forms)
+takes GPU tensors only.
 """
 from __future__ import annotations
 
@@ -48,11 +55,18 @@ class Process:
 
     def __call__(self, kinetic_energy: float, recoil_energy: float, element: AtomicElement,
                  mass: float) -> float:
-        """Scalar form of the reference functor (one pair evaluated on the GPU)."""
-        dev = torch.device("cuda", torch.cuda.current_device())
-        k = torch.tensor([float(kinetic_energy)], dtype=torch.float64, device=dev)
-        q = torch.tensor([float(recoil_energy)], dtype=torch.float64, device=dev)
-        return float(map(self)(k, q, element, mass).item())
+        """Scalar form of the reference functor: one pair, evaluated on the CPU."""
+        k = ctypes.c_double(float(kinetic_energy))
+        q = ctypes.c_double(float(recoil_energy))
+        r = ctypes.c_double(0.0)
+        outs = [None] * 4
+        outs[self.id] = ctypes.cast(ctypes.pointer(r), ctypes.c_void_p)
+        out = _lib.OutputsF64(*outs)
+        el = _element(element)
+        _lib.check(_lib.lib().noa_dcs_eval_f64_host(ctypes.cast(ctypes.pointer(k), ctypes.c_void_p),
+                                                    ctypes.cast(ctypes.pointer(q), ctypes.c_void_p), 1,
+                                                    ctypes.byref(el), float(mass), ctypes.byref(out), 1))
+        return r.value
 
 
 bremsstrahlung = Process(0, "bremsstrahlung", "src/noa/pms/physics.hh:114-153, dcs.hh:134-140")
@@ -66,14 +80,15 @@ def _element(el) -> _lib.Element:
     return _lib.Element(float(el.A), float(el.I), int(el.Z))
 
 
-def _check_inputs(K: torch.Tensor, q: torch.Tensor) -> None:
+def _check_inputs(K: torch.Tensor, q: torch.Tensor, allow_cpu: bool = False) -> None:
     for name, t in (("kinetic_energies", K), ("recoil_energies", q)):
         if not isinstance(t, torch.Tensor):
             raise TypeError(f"{name} must be a torch.Tensor")
         if t.dtype != torch.float64:
             raise ValueError(f"{name} must be float64 (got {t.dtype})")
-        if t.device.type != "cuda":
-            raise ValueError(f"{name} must be on a GPU device (got {t.device}); there is no CPU path")
+        if t.device.type != "cuda" and not (allow_cpu and t.device.type == "cpu"):
+            raise ValueError(f"{name} must be on a GPU device (got {t.device})"
+                             + ("" if allow_cpu else "; this form has no CPU path"))
         if not t.is_contiguous():
             raise ValueError(f"{name} must be contiguous")
     if q.numel() != K.numel():
@@ -106,13 +121,19 @@ def _stream(K: torch.Tensor):
 
 
 def eval_f64(K: torch.Tensor, q: torch.Tensor, element: AtomicElement, mass: float,
-             brems=None, pair=None, photo=None, ion=None) -> None:
-    """One fused launch writing every non-None output (FP64)."""
-    _check_inputs(K, q)
+             brems=None, pair=None, photo=None, ion=None, threads: int = 0) -> None:
+    """FP64 evaluation writing every non-None output: one fused launch for
+    GPU tensors; for CPU tensors the host build, serial for threads == 1,
+    the OpenMP default team for 0, k threads for k > 1."""
+    _check_inputs(K, q, allow_cpu=True)
     for r in (brems, pair, photo, ion):
         _check_output(r, K, torch.float64)
     out = _lib.OutputsF64(*(_ptr(r) for r in (brems, pair, photo, ion)))
     el = _element(element)
+    if K.device.type == "cpu":
+        _lib.check(_lib.lib().noa_dcs_eval_f64_host(_ptr(K), _ptr(q), K.numel(), ctypes.byref(el), float(mass),
+                                                    ctypes.byref(out), int(threads)))
+        return
     with torch.cuda.device(K.device):
         _lib.check(_lib.lib().noa_dcs_eval_f64(_ptr(K), _ptr(q), K.numel(), ctypes.byref(el), float(mass),
                                                ctypes.byref(out), _stream(K)))
@@ -137,8 +158,7 @@ def _process(f) -> Process:
     return f
 
 
-def vmap(dcs_func: Process):
-    """dcs.hh:35-48: returns f(result, K, q, element, mass) writing in place."""
+def _vmap(dcs_func: Process, threads: int):
     p = _process(dcs_func)
 
     def run(result: torch.Tensor, kinetic_energies: torch.Tensor, recoil_energies: torch.Tensor,
@@ -146,34 +166,68 @@ def vmap(dcs_func: Process):
         outs = [None] * 4
         outs[p.id] = result
         _check_output(result, kinetic_energies, torch.float64)
-        eval_f64(kinetic_energies, recoil_energies, element, mass, *outs)
+        eval_f64(kinetic_energies, recoil_energies, element, mass, *outs, threads=threads)
     return run
 
 
-def map(dcs_func: Process):  # noqa: A001  (the reference's name)
-    """dcs.hh:50-60: returns f(K, q, element, mass) -> new tensor like K."""
+def _map(dcs_func: Process, threads: int):
     p = _process(dcs_func)
 
     def run(kinetic_energies: torch.Tensor, recoil_energies: torch.Tensor, element: AtomicElement,
             mass: float) -> torch.Tensor:
         result = torch.zeros_like(kinetic_energies)
-        vmap(p)(result, kinetic_energies, recoil_energies, element, mass)
+        _vmap(p, threads)(result, kinetic_energies, recoil_energies, element, mass)
         return result
     return run
 
 
-pvmap = vmap   # dcs.hh:62-75 (OpenMP form); on the GPU the same kernels
-pmap = map     # dcs.hh:77-87
+def vmap(dcs_func: Process):
+    """dcs.hh:35-48: returns f(result, K, q, element, mass) writing in place
+    (serial on CPU tensors)."""
+    return _vmap(dcs_func, 1)
+
+
+def map(dcs_func: Process):  # noqa: A001  (the reference's name)
+    """dcs.hh:50-60: returns f(K, q, element, mass) -> new tensor like K
+    (serial on CPU tensors)."""
+    return _map(dcs_func, 1)
+
+
+def pvmap(dcs_func: Process):
+    """dcs.hh:62-75: vmap with an OpenMP team on CPU tensors (the default
+    team size, OMP_NUM_THREADS); on GPU tensors the same kernels as vmap."""
+    return _vmap(dcs_func, 0)
+
+
+def pmap(dcs_func: Process):
+    """dcs.hh:77-87: map with an OpenMP team on CPU tensors."""
+    return _map(dcs_func, 0)
 
 
 def _hip_namespace() -> SimpleNamespace:
     ns = {}
+
+    def gpu_vmap(p):
+        f = vmap(p)
+
+        def run(result, kinetic_energies, recoil_energies, element, mass) -> None:
+            _check_inputs(kinetic_energies, recoil_energies)
+            f(result, kinetic_energies, recoil_energies, element, mass)
+        return run
+
+    def gpu_map(p):
+        def run(kinetic_energies, recoil_energies, element, mass):
+            _check_inputs(kinetic_energies, recoil_energies)
+            return map(p)(kinetic_energies, recoil_energies, element, mass)
+        return run
+
     for p in PROCESSES:
-        ns[f"vmap_{p.name}"] = vmap(p)
-        ns[f"map_{p.name}"] = map(p)
+        ns[f"vmap_{p.name}"] = gpu_vmap(p)
+        ns[f"map_{p.name}"] = gpu_map(p)
 
     def vmap_all(results, kinetic_energies, recoil_energies, element, mass) -> None:
         """Fused 4-process launch into results = (brems, pair, photo, ion)."""
+        _check_inputs(kinetic_energies, recoil_energies)
         eval_f64(kinetic_energies, recoil_energies, element, mass, *results)
 
     def map_all(kinetic_energies, recoil_energies, element, mass):

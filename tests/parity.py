@@ -25,3 +25,20 @@ def compare(got: np.ndarray, exp: np.ndarray) -> dict:
         "nonzero": int(nz.sum()),
         "finite": finite,
     }
+
+
+def assert_parity(got, exp, tol: float, what: str = "") -> dict:
+    """NaN and infinite entries equal position for position, the zero
+    pattern exact, and the relative error of the other entries <= tol."""
+    got = np.asarray(got, dtype=np.float64)
+    exp = np.asarray(exp, dtype=np.float64)
+    assert got.shape == exp.shape, (what, got.shape, exp.shape)
+    gn, en = np.isnan(got), np.isnan(exp)
+    assert np.array_equal(gn, en), (what, "NaN pattern", int((gn != en).sum()))
+    gi, ei = np.isinf(got), np.isinf(exp)
+    assert np.array_equal(gi, ei) and np.array_equal(got[gi], exp[ei]), (what, "inf pattern")
+    fin = ~(gn | gi)
+    c = compare(got[fin], exp[fin])
+    assert c["zero_mismatch"] == 0, (what, c)
+    assert c["max_rel"] <= tol, (what, c)
+    return c

@@ -35,7 +35,9 @@ def fm(tmp_path_factory):
         y = np.empty_like(a)
         lib.fm_div_rn_batch(a.ctypes.data_as(D), b.ctypes.data_as(D), y.ctypes.data_as(D), a.size)
         return y
-    return {"log": lambda x: call(lib.fm_log_batch, x), "exp": lambda x: call(lib.fm_exp_batch, x), "div": div}
+    lib.fm_exp_fin_batch.argtypes = [D, D, ctypes.c_long]
+    return {"log": lambda x: call(lib.fm_log_batch, x), "exp": lambda x: call(lib.fm_exp_batch, x), "div": div,
+            "exp_fin": lambda x: call(lib.fm_exp_fin_batch, x)}
 
 
 def ulp_err(got: np.ndarray, exact: list) -> np.ndarray:
@@ -132,3 +134,16 @@ def test_div_rn_is_the_division(fm):
     want = a / b
     bad = np.flatnonzero(got != want)
     assert bad.size == 0, (bad.size, a[bad[:4]], b[bad[:4]])
+
+
+def test_exp_fin_over_its_stated_range(fm):
+    """exp_fin (no NaN test, no clamp) equals exp over the physical range and
+    still overflows to inf / underflows to 0 at the ends of its stated range
+    |x| < 2^31 ln2 / 128, where its 32-bit k has not wrapped."""
+    rng = np.random.default_rng(11)
+    x = np.concatenate([rng.uniform(-745.0, 709.7, 100000), rng.uniform(-1.0, 1.0, 10000)])
+    np.testing.assert_array_equal(fm["exp_fin"](x), fm["exp"](x))
+    edge = 2.0 ** 31 * np.log(2.0) / 128.0 * (1 - 1e-6)
+    with np.errstate(all="ignore"):
+        got = fm["exp_fin"](np.array([edge, -edge, 800.0, -800.0, 1.0e6, -1.0e6, np.nan]))
+    assert np.all(got[[0, 2, 4]] == np.inf) and np.all(got[[1, 3, 5]] == 0.0) and np.isnan(got[6])

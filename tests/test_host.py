@@ -52,11 +52,15 @@ def test_element_batch():
     assert K.shape == (int(offs[3]),)
 
 
-def test_cpu_tensors_are_rejected():
+def test_gpu_only_forms_reject_cpu_tensors():
     K = torch.tensor([1.0, 10.0], dtype=torch.float64)
     q = torch.tensor([0.1, 1.0], dtype=torch.float64)
     with pytest.raises(ValueError, match="no CPU path"):
-        dcs.map(dcs.bremsstrahlung)(K, q, STANDARD_ROCK, MUON_MASS)
+        dcs.hip.map_bremsstrahlung(K, q, STANDARD_ROCK, MUON_MASS)
+    with pytest.raises(ValueError, match="no CPU path"):
+        dcs.hip.map_all_f32mixed(K, q, STANDARD_ROCK, MUON_MASS)
+    with pytest.raises(ValueError, match="no CPU path"):
+        dcs.hip.vmap_all(tuple(torch.zeros_like(K) for _ in range(4)), K, q, STANDARD_ROCK, MUON_MASS)
     with pytest.raises(ValueError, match="float64"):
         dcs.map(dcs.pair_production)(K.float(), q, STANDARD_ROCK, MUON_MASS)
     with pytest.raises(TypeError):
@@ -72,7 +76,8 @@ def test_reference_names_are_exposed():
     for p in dcs.PROCESSES:
         assert callable(getattr(dcs.hip, f"vmap_{p.name}")) and callable(getattr(dcs.hip, f"map_{p.name}"))
     assert dcs.cuda.map_bremsstrahlung is dcs.hip.map_bremsstrahlung
-    assert dcs.pvmap is dcs.vmap and dcs.pmap is dcs.map
+    for f in (dcs.vmap, dcs.map, dcs.pvmap, dcs.pmap):
+        assert callable(f(dcs.ionisation))
 
 
 def test_relative_error_metric():
