@@ -9,7 +9,8 @@ DCS values). The same JSON line also reports the mixed-precision FP32 form,
 the batched 48-element persistent launch and (when trained) the surrogate,
 each with its max relative error against the FP64 GPU result over every pair,
 the roofline of the FP64 kernel against the FP64 vector peak, and the CPU
-baseline (the reference's OpenMP path on this host's cores).
+baseline (the restated oracle, the reference's OpenMP path, on this host's
+cores).
 """
 from __future__ import annotations
 
@@ -26,6 +27,10 @@ CPU_THREADS = 16  # the host CPU share of one GPU on the benchmark boxes
 os.environ.setdefault("OMP_NUM_THREADS", str(CPU_THREADS))
 
 FP64_VECTOR_PEAK_TFLOPS = 78.6
+FP32_VECTOR_PEAK_TFLOPS = 157.3
+# wave64 VALU instruction issue: 256 CUs x 4 SIMDs x 2.4e9 cycles / 4 cycles per
+# wave64 instruction (16 FP64 lanes per SIMD per cycle)
+VALU_WAVE_ISSUE_PER_S = 256 * 4 * 2.4e9 / 4
 METRIC = "DCS evals/sec (4 processes) FP64 vs FP32 vs surrogate at stated max rel err"
 
 
@@ -91,21 +96,40 @@ def read_traffic(kernel_prefix: str):
     return vals.get("hbm")
 
 
-def read_executed_flops():
-    """Executed FP64 flops per pair (2 x FMA + ADD + MUL + TRANS) from the
-    committed rocprofv3 VALU counter summary, or None."""
-    path = os.path.join(ROOT, "profiles", "dcs_eval_f64_2e24_valu.txt")
-    if not os.path.exists(path):
-        return None
+def read_counters(name: str) -> dict:
+    """Per-pair lane instruction counts ("SQ_...: <x> per dispatch, <y> per
+    pair") from a committed rocprofv3 --pmc summary under profiles/."""
+    path = os.path.join(ROOT, "profiles", name)
     per = {}
-    for line in open(path):
-        if line.startswith("SQ_INSTS_VALU_") and "per pair" in line:
-            name = line.split(":")[0].strip()
-            per[name] = float(line.split(",")[1].split()[0])
-    need = ("SQ_INSTS_VALU_FMA_F64", "SQ_INSTS_VALU_ADD_F64", "SQ_INSTS_VALU_MUL_F64", "SQ_INSTS_VALU_TRANS_F64")
-    if not all(k in per for k in need):
-        return None
-    return 2 * per[need[0]] + per[need[1]] + per[need[2]] + per[need[3]]
+    if os.path.exists(path):
+        for line in open(path):
+            if line.startswith("SQ_INSTS_") and "per pair" in line:
+                per[line.split(":")[0].strip()] = float(line.split(",")[1].split()[0])
+    return per
+
+
+def valu_roofline(n: int, kernel_ms: float, peak_tflops: float, counters: dict, flops_pair: float,
+                  kind: str) -> dict:
+    """Roofline of a VALU-bound DCS kernel: the algorithmic flops (the cost
+    model's count per pair x pairs) per average launch duration against the
+    vector peak, with the executed flops and the VALU issue rate of the
+    committed counter summary of the same kernel beside."""
+    s = kernel_ms * 1e-3
+    achieved = flops_pair * n / s / 1e12
+    out = {"bound": "valu", "achieved": achieved, "peak": peak_tflops, "unit": "TFLOP/s",
+           "frac": achieved / peak_tflops, "kernel_ms": kernel_ms, "algorithmic_flops_per_pair": flops_pair}
+    t = kind
+    need = (f"SQ_INSTS_VALU_FMA_{t}", f"SQ_INSTS_VALU_ADD_{t}", f"SQ_INSTS_VALU_MUL_{t}",
+            f"SQ_INSTS_VALU_TRANS_{t}")
+    if all(k in counters for k in need):
+        ex = 2 * counters[need[0]] + counters[need[1]] + counters[need[2]] + counters[need[3]]
+        out["executed_flops_per_pair"] = ex
+        out["executed_frac"] = ex * n / s / 1e12 / peak_tflops
+    if "SQ_INSTS_VALU" in counters:
+        out["valu_instructions_per_pair"] = counters["SQ_INSTS_VALU"]
+        if kind == "F64":  # FP32 instructions can issue faster than 4 cycles per wave64
+            out["valu_issue_frac"] = counters["SQ_INSTS_VALU"] * n / 64 / s / VALU_WAVE_ISSUE_PER_S
+    return out
 
 
 def cpu_model() -> str:
@@ -119,44 +143,52 @@ def cpu_model() -> str:
 
 
 def cpu_baseline(n_total: int, min_seconds: float = 10.0) -> dict:
-    """The reference's OpenMP path (oracle/_ref, compiled from the reference
-    headers) or, where it is absent, the restated oracle, on this host: the
-    whole workload, repeated until at least `min_seconds` of CPU work, plus a
-    one-thread figure on its first 2^20 pairs."""
+    """The restated oracle (oracle/, the reference's FP64 CPU path in the
+    reference's operation order, with its OpenMP loop per process as
+    dcs::pvmap runs it) on this host: the whole workload, repeated until at
+    least `min_seconds` of CPU work, plus a one-thread figure on its first
+    2^20 pairs. Its speed matches the compiled reference's within a few
+    percent on the same host (measured where the reference sources exist).
+    Beside it, the library's own CPU-tensor path (dcs.pmap on CPU tensors)
+    on the first 2^22 pairs."""
     import numpy as np
+    import torch
 
     import oracle
-    from noa_amd import workload
+    from noa_amd import dcs, workload
     from noa_amd.physics import MUON_MASS, STANDARD_ROCK
-    ref = oracle.load_ref()
-    kind = "reference" if ref is not None else "port"
     n = n_total
     K, q = workload.pairs(n, seed=0)
 
-    def run(m, parallel=True):
-        if kind == "reference":
-            for p in range(4):
-                oracle.ref_map(p, K[:m], q[:m], STANDARD_ROCK, MUON_MASS, parallel=parallel)
-        else:
-            oracle.dcs_map_all(K[:m], q[:m], STANDARD_ROCK, MUON_MASS, threads=CPU_THREADS if parallel else 1)
+    def run(m, threads):
+        for p in range(4):
+            oracle.dcs_map(p, K[:m], q[:m], STANDARD_ROCK, MUON_MASS, threads=threads)
 
-    run(1 << 14)  # warm-up (thread team, page faults)
+    run(1 << 14, CPU_THREADS)  # warm-up (thread team, page faults)
     reps, t = 0, time.perf_counter()
     while True:
-        run(n)
+        run(n, CPU_THREADS)
         reps += 1
         dt = time.perf_counter() - t
         if dt >= min_seconds or reps >= 64:
             break
     n1 = min(n, 1 << 20)
     t1 = time.perf_counter()
-    run(n1, parallel=False)
+    run(n1, 1)
     dt1 = time.perf_counter() - t1
-    return {"value": n * reps / dt, "unit": "evals/s", "cores": int(os.environ.get("OMP_NUM_THREADS", CPU_THREADS)),
-            "kind": kind,
+    n2 = min(n, 1 << 22)
+    Kt, qt = torch.from_numpy(K[:n2]), torch.from_numpy(q[:n2])
+    t2 = time.perf_counter()
+    for p in dcs.PROCESSES:
+        dcs.pmap(p)(Kt, qt, STANDARD_ROCK, MUON_MASS)
+    dt2 = time.perf_counter() - t2
+    return {"value": n * reps / dt, "unit": "evals/s", "cores": CPU_THREADS, "kind": "port",
             "sample": f"the whole {n}-pair workload (2^{int(np.log2(n))}) x {reps} repetitions, all 4 processes, "
                       f"one OpenMP map per process as dcs::pvmap does, {dt:.2f} s",
             "one_thread": {"value": n1 / dt1, "sample": f"first {n1} pairs, serial dcs::vmap, {dt1:.2f} s"},
+            "library_cpu_path": {"value": n2 / dt2, "cores": int(os.environ.get("OMP_NUM_THREADS", CPU_THREADS)),
+                                 "sample": f"first {n2} pairs, dcs.pmap per process on CPU tensors (the host build "
+                                           f"of the kernels' FP64 physics), {dt2:.2f} s"},
             "cpu_model": cpu_model(), "host_cpus": os.cpu_count()}
 
 
@@ -228,9 +260,10 @@ def main() -> None:
 
     cm = json.load(open(os.path.join(ROOT, "tests", "golden", "costmodel.json")))
     flops_pair = cm["fp64_flops_per_pair_total"]
-    achieved = flops_pair * n / (kern_ms * 1e-3) / 1e12
-    traffic = read_traffic("dcs_eval_kernel")
-    executed = read_executed_flops()
+    roof = valu_roofline(n, kern_ms, FP64_VECTOR_PEAK_TFLOPS, read_counters("dcs_eval_f64_2e24_valu.txt"),
+                         flops_pair, "F64")
+    roof.update({"traffic": read_traffic("dcs_eval_kernel"), "kernel": "dcs_eval_kernel<F64,2048>",
+                 "kernel_ms_median": kern_ms_median, "flop_rule": cm["rule"]})
     result = {
         "metric": METRIC,
         "value": value,
@@ -251,25 +284,7 @@ def main() -> None:
             "n_pairs": n,
             "parallelism": "single GPU",
         },
-        "roofline": {
-            "bound": "valu",
-            "achieved": achieved,
-            "peak": FP64_VECTOR_PEAK_TFLOPS,
-            "unit": "TFLOP/s",
-            "frac": achieved / FP64_VECTOR_PEAK_TFLOPS,
-            "traffic": traffic,
-            "kernel": "dcs_eval_kernel<F64,2048>",
-            "kernel_ms": kern_ms,
-            "kernel_ms_median": kern_ms_median,
-            "algorithmic_flops_per_pair": flops_pair,
-            # the algorithmic count prices each log/exp at ocml's instruction
-            # cost; the kernel's table-driven versions take about a fifth of
-            # that, so frac can pass 1. The executed figure comes from the
-            # committed VALU counter profile of this kernel.
-            "executed_flops_per_pair": executed,
-            "executed_frac": (executed * n / (kern_ms * 1e-3) / 1e12 / FP64_VECTOR_PEAK_TFLOPS
-                              if executed else None),
-        },
+        "roofline": roof,
     }
 
     if not args.no_extras:
@@ -279,7 +294,10 @@ def main() -> None:
         w32, k32 = timed_launches(lambda: dcs.eval_f32mixed(K, q, STANDARD_ROCK, MUON_MASS, *o32),
                                   args.steps, args.warmup, stream)
         forms["f32mixed"] = {"evals_per_s": n * args.steps / w32, "kernel_ms": k32,
-                             "speedup_vs_f64": wall / w32, "max_rel_err_vs_f64": max_rel_vs(o32, ref)}
+                             "speedup_vs_f64": wall / w32, "max_rel_err_vs_f64": max_rel_vs(o32, ref),
+                             "roofline": valu_roofline(n, k32, FP32_VECTOR_PEAK_TFLOPS,
+                                                       read_counters("dcs_eval_f32mixed_2e24_valu.txt"),
+                                                       flops_pair, "F32")}
         del o32
         # per-process rates: one output requested, the other processes skipped
         per = {}

@@ -100,7 +100,17 @@ struct F32MN {
         return float(c.Zd * Phi_n + Phi_e);
     }
 
+    // The range the FP32 arithmetic is laid out for: 1e-20 <= K <= 1e12 GeV,
+    // 1e-30 <= q <= 1e12 GeV and q >= 1e-30 E, where the FP32 products and
+    // quotients below (E^2, 1 / nu, q^2, ...) stay finite and normal. Pairs
+    // outside it (NaN fails the test) take the FP64 physics: the closed forms
+    // directly, the integrals through the redo list.
+    static __device__ __forceinline__ bool in_range(double K, double q, const DcsConsts& c) {
+        return (K >= 1e-20) && (K <= 1e12) && (q >= 1e-30) && (q <= 1e12) && (q >= 1e-30 * (K + c.mass));
+    }
+
     static __device__ __forceinline__ float brems(double K, double q, const DcsConsts& c, const fm::Tab& t) {
+        if (!in_range(K, q, c)) return float(F64::brems(K, q, c, t));
         const float me = float(ELECTRON_MASS);
         const float sqrte = float(SQRTE_BREMS);
         const double E = K + c.mass;
@@ -125,13 +135,24 @@ struct F32MN {
         return float(c.f_brems_c) * S * F;
     }
 
-    static __device__ __forceinline__ float ion(double K, double q, const DcsConsts& c, const fm::Tab& t) {
+    // The ionisation exits in the reference's FP64 arithmetic (dcs.hh:416-423,
+    // no contraction), so that a pair is zero here exactly when it is there.
+    static __device__ __forceinline__ bool ion_open(double K, double q, const DcsConsts& c, double& P2, double& E,
+                                                    double& Wmax) {
+#pragma clang fp contract(off)
         const double me = ELECTRON_MASS;
-        const double P2 = K * (K + c.i_2m);
-        const double E = K + c.mass;
-        const double Wmax = 2. * me * P2 / (c.i_m2 + me * (me + 2. * E));
-        if ((Wmax < X_FRACTION * K) || (q > Wmax)) return 0.f;
-        if (q <= c.i_Wmin) return 0.f;
+        P2 = K * (K + c.i_2m);
+        E = K + c.mass;
+        Wmax = 2. * me * P2 / (c.i_m2 + me * (me + 2. * E));
+        if ((Wmax < X_FRACTION * K) || (q > Wmax)) return false;
+        return !(q <= c.i_Wmin);
+    }
+
+    static __device__ __forceinline__ float ion(double K, double q, const DcsConsts& c, const fm::Tab& t) {
+        if (!in_range(K, q, c)) return float(F64::ion(K, q, c, t));
+        const double me = ELECTRON_MASS;
+        double P2, E, Wmax;
+        if (!ion_open(K, q, c, P2, E, Wmax)) return 0.f;
         const float qf = float(q), P2f = float(P2), Ef = float(E), Wf = float(Wmax);
         const float m2f = float(c.i_m2);
         // a0 + (a1 + a2 / q) / q with a1 + a2 / q = ((Wmax - q) / Wmax + m^2 / P^2) / q
@@ -156,6 +177,10 @@ struct F32MN {
     static constexpr float PAIR_REDO_TOL = 2e-6f;
 
     static __device__ float pair(double K, double q, const DcsConsts& c, bool& redo, const fm::Tab& t) {
+        if (!in_range(K, q, c)) {
+            redo = true;
+            return 0.f;
+        }
         // FP64 set-up: the integration bound exactly as dcs.hh:169-176
         double tmin_d;
         {
@@ -252,7 +277,8 @@ struct F32MN {
     }
 
     static __device__ float photo(double K, double q, const DcsConsts& c, bool& redo, const fm::Tab& t) {
-        redo = false;
+        redo = !in_range(K, q, c);
+        if (redo) return 0.f;
         const double M = PHOTO_M;
         const double mpi = PHOTO_MPI;
         const double E = K + c.mass;

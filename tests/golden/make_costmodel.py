@@ -3,16 +3,21 @@
     python tests/golden/make_costmodel.py
 
 Two inputs, both measured here and frozen in the JSON:
- 1. FP64 flops of one device libm call (log, exp, log10, sqrt, pow) and of one
-    FP64 division, counted in the gfx950 assembly that hipcc generates for
-    libm_cost.hip: v_fma/v_fmac/v_div_fmas count 2, every other value-producing
-    FP64 VALU instruction (add, mul, ldexp, frexp, rcp, rsq, rndne, div_scale,
-    div_fixup) counts 1, compares and conversions count 0;
- 2. the branch-accurate tally of +,-,*,/,sqrt and libm calls per pair of each
-    process, from the CPU oracle replaying its arithmetic on an op-counting
-    scalar over 2^18 standard-rock pairs (seed 0). Per-element set-up (work on
-    the element and mass alone) is excluded: it is done once per launch.
-flops per pair of a process = sum over op kinds of tally * weight.
+ 1. the branch-accurate tally of +,-,*,/,sqrt and transcendental calls per
+    pair of each process, from the CPU oracle replaying its arithmetic on an
+    op-counting scalar over 2^18 standard-rock pairs (seed 0). Per-element
+    set-up (work on the element and mass alone) is excluded: it is done once
+    per launch;
+ 2. the flop weight of one log and one exp: the FP64 flops of the project's
+    own table-driven log / exp (noa_amd/csrc/fastmath.hh, their fast-path
+    forms log_pos / exp_fin, which the kernels call), counted in the gfx950
+    assembly that hipcc generates for libm_cost.hip: v_fma/v_fmac count 2,
+    every other value-producing FP64 VALU instruction counts 1, compares and
+    conversions count 0. log10 is a log and one multiply.
++, -, *, / and sqrt count 1 flop each (an FMA 2). flops per pair of a process
+= sum over op kinds of tally * weight. The FP64 flops of ocml's log, exp,
+log10, sqrt, pow and of the IEEE division sequence are recorded beside, for
+reference only (the kernels do not call them on their fast paths).
 """
 from __future__ import annotations
 
@@ -26,7 +31,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 
-KERNELS = {"log": "k_log", "exp": "k_exp", "log10": "k_log10", "sqrt": "k_sqrt", "div": "k_div", "pow": "k_pow"}
+KERNELS = {"log": "k_log", "exp": "k_exp", "log10": "k_log10", "sqrt": "k_sqrt", "div": "k_div", "pow": "k_pow",
+           "fm_log_pos": "k_fm_log_pos", "fm_exp_fin": "k_fm_exp_fin"}
 
 
 def libm_weights() -> dict:
@@ -65,14 +71,16 @@ def tallies(n: int = 1 << 18) -> dict:
 
 def main() -> None:
     w = libm_weights()
-    weights = {"add": 1, "mul": 1, "cmp": 0, "div": w["div"]["flops"], "sqrt": w["sqrt"]["flops"],
-               "log": w["log"]["flops"], "exp": w["exp"]["flops"], "log10": w["log10"]["flops"],
-               "pow": w["pow"]["flops"]}
     t = tallies()
+    assert all(ops["pow"] == 0 for ops in t.values()), "pow is per-element set-up only"
+    weights = {"add": 1, "mul": 1, "cmp": 0, "div": 1, "sqrt": 1,
+               "log": w["fm_log_pos"]["flops"], "exp": w["fm_exp_fin"]["flops"],
+               "log10": w["fm_log_pos"]["flops"] + 1, "pow": 0}
     per_process = {p: sum(v * weights[k] for k, v in ops.items()) for p, ops in t.items()}
     doc = {
         "workload": "standard rock, muon, K log-uniform [1e-3, 1e6] GeV, nu log-uniform [1e-6, 1), seed 0, 2^18 pairs",
-        "libm_gfx950": w,
+        "rule": "+ - * / sqrt = 1 flop, FMA = 2; log, exp at the flops of the project's own log_pos / exp_fin",
+        "transcendental_flops_gfx950": w,
         "weights_fp64_flops": weights,
         "ops_per_pair": t,
         "fp64_flops_per_pair": per_process,
