@@ -117,6 +117,25 @@ int noa_dcs_recoil_integrals_f64(const double* K, int64_t n, double xlow, const 
                                  const noa_dcs_outputs_f64* cel, void* workspace, int64_t workspace_bytes,
                                  hipStream_t stream);
 
+/*
+ * The same recoil integrals with no allocation and no workspace argument.
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc). It calls no hipMalloc / hipFree and no allocating library
+ * routine; its node pairs and DCS planes live in a file-scope device array
+ * of noa_dcs_recoil_noalloc_scratch_bytes() (32 MiB) that each GPU holds
+ * until the process ends. Any n is processed, in chunks of
+ * scratch / (48 ceil(min_points / 6) * 6) kinetic energies (3883 at
+ * min_points = 180); a min_points whose single kinetic energy would not fit
+ * returns NOA_DCS_ERANGE. Results equal noa_dcs_recoil_integrals_f64's bit
+ * for bit. Every call on a GPU shares that GPU's array: calls that overlap
+ * on one GPU (two streams, or two host threads) race; serialise them.
+ */
+int64_t noa_dcs_recoil_noalloc_scratch_bytes(void);
+int noa_dcs_recoil_integrals_f64_noalloc(const double* K, int64_t n, double xlow, const noa_dcs_element* element,
+                                         double mass, int32_t min_points, const noa_dcs_outputs_f64* del,
+                                         const noa_dcs_outputs_f64* cel, hipStream_t stream);
+
 /* ---- Coulomb scattering (src/noa/pms/dcs.hh:499-950) ----
  * Arrays are row-major like the reference's tensors: fCM [n][2],
  * screening [n][9], G [n][2]; for a mixture of nel elements the per-element

@@ -89,6 +89,10 @@ SIGNATURES = {
     "noa_dcs_recoil_integrals_f64": (ctypes.c_int, [_P, _I64, _D, ctypes.POINTER(Element), _D, _I32,
                                                     ctypes.POINTER(OutputsF64), ctypes.POINTER(OutputsF64), _P,
                                                     _I64, _P]),
+    "noa_dcs_recoil_noalloc_scratch_bytes": (ctypes.c_int64, []),
+    "noa_dcs_recoil_integrals_f64_noalloc": (ctypes.c_int, [_P, _I64, _D, ctypes.POINTER(Element), _D, _I32,
+                                                            ctypes.POINTER(OutputsF64), ctypes.POINTER(OutputsF64),
+                                                            _P]),
     "noa_dcs_mlp_num_params": (ctypes.c_int64, []),
     "noa_dcs_mlp_create": (ctypes.c_int, [_U64, _I32, ctypes.POINTER(_P)]),
     "noa_dcs_mlp_destroy": (ctypes.c_int, [_P]),

@@ -15,6 +15,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <mutex>
 
 #include "capi_util.hh"
 #include "dcs_consts.hh"
@@ -29,6 +30,14 @@ static __constant__ double GL6_X[6] = {0.03376524, 0.16939531, 0.38069041, 0.619
 static __constant__ double GL6_W[6] = {0.08566225, 0.18038079, 0.23395697, 0.23395697, 0.18038079, 0.08566225};
 
 constexpr int RECOIL_CHUNK = 65536;  // K values per pass
+
+// Scratch of the no-allocation entry point noa_dcs_recoil_integrals_f64_noalloc:
+// file-scope device memory, one copy per GPU the library runs on, held until
+// the process ends. Host code never uses this name as an address (on the host
+// it is a stand-in for the symbol): it takes the device address once per
+// device from hipGetSymbolAddress.
+constexpr int64_t RECOIL_SCRATCH_BYTES = int64_t(32) << 20;
+static __device__ __align__(256) unsigned char scratch[RECOIL_SCRATCH_BYTES];
 
 struct RecoilGrid {
     double xlow;
@@ -139,45 +148,33 @@ namespace {
 
 int64_t nodes_of(int32_t min_points) { return int64_t((uint32_t(min_points) + 5u) / 6u) * 6; }
 
-}  // namespace
+// Bytes of scratch per kinetic energy: the node pairs (K, q) and the four
+// DCS planes, 6 doubles per node.
+int64_t bytes_per_K(int32_t min_points) { return nodes_of(min_points) * int64_t(6 * sizeof(double)); }
 
-extern "C" {
-
-int64_t noa_dcs_recoil_workspace_bytes(int64_t n, int32_t min_points) {
-    if (n <= 0 || min_points < 1) return 0;
-    return std::min<int64_t>(n, RECOIL_CHUNK) * nodes_of(min_points) * int64_t(6 * sizeof(double));
-}
-
-int noa_dcs_recoil_integrals_f64(const double* K, int64_t n, double xlow, const noa_dcs_element* element,
-                                 double mass, int32_t min_points, const noa_dcs_outputs_f64* del,
-                                 const noa_dcs_outputs_f64* cel, void* workspace, int64_t workspace_bytes,
-                                 hipStream_t stream) {
+int check_recoil_args(const double* K, int64_t n, double xlow, const noa_dcs_element* element, double mass,
+                      int32_t min_points) {
     if (int rc = check_element(element, mass)) return rc;
     if (n < 0) return fail(NOA_DCS_EINVAL, "n must be >= 0");
     if (min_points < 1 || min_points > 6000) return fail(NOA_DCS_EINVAL, "min_points must be in [1, 6000]");
     if (!(xlow > 0.0 && xlow < 1.0)) return fail(NOA_DCS_EINVAL, "xlow must be in (0, 1) (got %g)", xlow);
-    if (n == 0) return ok();
-    if (!K) return fail(NOA_DCS_EINVAL, "K is NULL");
-    const noa_dcs_outputs_f64 none{nullptr, nullptr, nullptr, nullptr};
-    const noa_dcs_outputs_f64& D = del ? *del : none;
-    const noa_dcs_outputs_f64& C = cel ? *cel : none;
-    bool want[4];
-    const double* dd[4] = {D.brems, D.pair, D.photo, D.ion};
-    const double* cc[4] = {C.brems, C.pair, C.photo, C.ion};
-    bool any = false;
-    for (int p = 0; p < 4; p++) any |= (want[p] = dd[p] || cc[p]);
-    if (!any) return ok();
-    if (!workspace || workspace_bytes < noa_dcs_recoil_workspace_bytes(n, min_points))
-        return fail(NOA_DCS_EINVAL, "workspace must hold noa_dcs_recoil_workspace_bytes(n, min_points) bytes");
+    if (n > 0 && !K) return fail(NOA_DCS_EINVAL, "K is NULL");
+    return NOA_DCS_OK;
+}
 
+// The passes over chunks of at most chunk_cap kinetic energies, with the
+// node pairs and DCS planes in `work` (chunk_cap * bytes_per_K bytes).
+int recoil_run(const double* K, int64_t n, double xlow, const noa_dcs_element* element, double mass,
+               int32_t min_points, const noa_dcs_outputs_f64& D, const noa_dcs_outputs_f64& C, double* work,
+               int64_t chunk_cap, hipStream_t stream) {
+    const bool want[4] = {D.brems || C.brems, D.pair || C.pair, D.photo || C.photo, D.ion || C.ion};
     const RecoilGrid g{xlow, int32_t((uint32_t(min_points) + 5u) / 6u), int32_t(nodes_of(min_points))};
-    const int64_t chunk_cap = std::min<int64_t>(n, RECOIL_CHUNK);
-    double* Kn = static_cast<double*>(workspace);
+    double* Kn = work;
     double* qn = Kn + chunk_cap * g.nodes;
     double* planes = qn + chunk_cap * g.nodes;
     const DcsConsts c = make_consts(element->A, element->I, element->Z, mass);
-    for (int64_t k0 = 0; k0 < n; k0 += RECOIL_CHUNK) {
-        const int64_t nK = std::min<int64_t>(RECOIL_CHUNK, n - k0);
+    for (int64_t k0 = 0; k0 < n; k0 += chunk_cap) {
+        const int64_t nK = std::min<int64_t>(chunk_cap, n - k0);
         const int64_t nn = nK * g.nodes;
         const unsigned grid = unsigned(std::min<int64_t>(65536, (nn + 255) / 256));
         hipLaunchKernelGGL(recoil_nodes_kernel, dim3(grid), dim3(256), 0, stream, K + k0, nK, g, Kn, qn);
@@ -194,6 +191,68 @@ int noa_dcs_recoil_integrals_f64(const double* K, int64_t n, double xlow, const 
         if (int rc = check_hip(hipGetLastError(), "recoil_reduce_kernel")) return rc;
     }
     return ok();
+}
+
+// Device address of `scratch` on the current device, looked up once per
+// device (each GPU holds its own copy of the array at its own address).
+int scratch_address(void** out) {
+    static std::mutex mu;
+    static void* addr[64] = {};
+    int dev = 0;
+    if (int rc = check_hip(hipGetDevice(&dev), "hipGetDevice")) return rc;
+    if (dev < 0 || dev >= 64) return fail(NOA_DCS_ERANGE, "device ordinal %d beyond 63", dev);
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[dev]) {
+        void* p = nullptr;
+        if (int rc = check_hip(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)"))
+            return rc;
+        addr[dev] = p;
+    }
+    *out = addr[dev];
+    return NOA_DCS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t noa_dcs_recoil_workspace_bytes(int64_t n, int32_t min_points) {
+    if (n <= 0 || min_points < 1) return 0;
+    return std::min<int64_t>(n, RECOIL_CHUNK) * nodes_of(min_points) * int64_t(6 * sizeof(double));
+}
+
+int noa_dcs_recoil_integrals_f64(const double* K, int64_t n, double xlow, const noa_dcs_element* element,
+                                 double mass, int32_t min_points, const noa_dcs_outputs_f64* del,
+                                 const noa_dcs_outputs_f64* cel, void* workspace, int64_t workspace_bytes,
+                                 hipStream_t stream) {
+    if (int rc = check_recoil_args(K, n, xlow, element, mass, min_points)) return rc;
+    const noa_dcs_outputs_f64 none{nullptr, nullptr, nullptr, nullptr};
+    const noa_dcs_outputs_f64& D = del ? *del : none;
+    const noa_dcs_outputs_f64& C = cel ? *cel : none;
+    if (n == 0 || !(D.brems || D.pair || D.photo || D.ion || C.brems || C.pair || C.photo || C.ion)) return ok();
+    if (!workspace || workspace_bytes < noa_dcs_recoil_workspace_bytes(n, min_points))
+        return fail(NOA_DCS_EINVAL, "workspace must hold noa_dcs_recoil_workspace_bytes(n, min_points) bytes");
+    return recoil_run(K, n, xlow, element, mass, min_points, D, C, static_cast<double*>(workspace),
+                      std::min<int64_t>(n, RECOIL_CHUNK), stream);
+}
+
+int64_t noa_dcs_recoil_noalloc_scratch_bytes(void) { return RECOIL_SCRATCH_BYTES; }
+
+int noa_dcs_recoil_integrals_f64_noalloc(const double* K, int64_t n, double xlow, const noa_dcs_element* element,
+                                         double mass, int32_t min_points, const noa_dcs_outputs_f64* del,
+                                         const noa_dcs_outputs_f64* cel, hipStream_t stream) {
+    if (int rc = check_recoil_args(K, n, xlow, element, mass, min_points)) return rc;
+    const noa_dcs_outputs_f64 none{nullptr, nullptr, nullptr, nullptr};
+    const noa_dcs_outputs_f64& D = del ? *del : none;
+    const noa_dcs_outputs_f64& C = cel ? *cel : none;
+    if (n == 0 || !(D.brems || D.pair || D.photo || D.ion || C.brems || C.pair || C.photo || C.ion)) return ok();
+    const int64_t chunk_cap = std::min<int64_t>(n, RECOIL_SCRATCH_BYTES / bytes_per_K(min_points));
+    if (chunk_cap < 1)
+        return fail(NOA_DCS_ERANGE, "min_points %d needs %lld scratch bytes per kinetic energy, more than the %lld held",
+                    min_points, (long long)bytes_per_K(min_points), (long long)RECOIL_SCRATCH_BYTES);
+    void* work = nullptr;
+    if (int rc = scratch_address(&work)) return rc;
+    return recoil_run(K, n, xlow, element, mass, min_points, D, C, static_cast<double*>(work), chunk_cap, stream);
 }
 
 }  // extern "C"
