@@ -90,6 +90,44 @@ int noa_dcs_eval_f64_host(const double* K, const double* q, int64_t n, const noa
                           double mass, const noa_dcs_outputs_f64* out, int32_t n_threads);
 
 /*
+ * PUMAS v1.2.1 DCS tabulation (the physics tables PUMAS builds from its DCS
+ * models). Replaces compute_dcs_table (src/noa/3rdparty/_pumas/pumas.c:
+ * 11693-11787; declared :1270) with its row fill dcs_tabulate_row
+ * (:11422-11507), for the default models (pumas.c:190-198): process 0 SSR
+ * bremsstrahlung, 1 SSR pair production, 2 DRSS photonuclear.
+ *
+ * noa_pumas_n_table_dcs(cutoff): the number of grid nodes n_x of a relative
+ * cutoff in (0, 1) (pumas.c:1548-1554), or a negative NOA_DCS_E* code
+ * (NOA_DCS_ERANGE beyond 192 nodes, i.e. cutoff below ~2e-12).
+ *
+ * noa_pumas_dcs_table: Z and A (n_elements each) are host arrays (the
+ * element invariants are computed on the host with its libm, as the
+ * reference computes them); the kinetic energies K (n_K, GeV) are a device
+ * array. Outputs (device, caller-owned), in PUMAS's own layouts
+ * (pumas.c:4804-4827):
+ *   table    [3][n_elements][n_K][2 n_x] floats: per node ln DCS and the
+ *            PCHIP slope over the first run of positive DCS values, 0
+ *            elsewhere (table_get_dcs);
+ *   x        [n_x] floats, the nodes ln(q / K) (table_DCS_x), or NULL;
+ *   envelope [3][n_elements][n_K][2] floats, or NULL: entry 0 the envelope
+ *            exponent (with the low-energy fill of pumas.c:11745-11761);
+ *            entry 1, the envelope maximum, is written as 0 (not computed);
+ *   table_f64 the table in FP64 before its float rounding, or NULL.
+ * Rows are not gated by the cross-section normalisation (pumas.c:11427-11434
+ * zeroes a row whose total cross section is 0): callers that hold it apply
+ * it. Work is enqueued on `stream`.
+ *
+ * noa_pumas_dcs_f64: the model DCS of one process (m^2/kg, the wrappers of
+ * pumas.c:11067-11112) over n device (K, q) pairs.
+ */
+int noa_pumas_n_table_dcs(double cutoff);
+int noa_pumas_dcs_table(const double* Z, const double* A, int32_t n_elements, double mass, double cutoff,
+                        const double* K, int32_t n_K, float* table, float* x, float* envelope, double* table_f64,
+                        hipStream_t stream);
+int noa_pumas_dcs_f64(int32_t process, double Z, double A, double mass, const double* K, const double* q, int64_t n,
+                      double* out, hipStream_t stream);
+
+/*
  * Mixed-precision evaluation: FP32 arithmetic with FP64 kept for the
  * kinematic differences (E - q, K - q, 1 - nu), the pair-production
  * integration bound and the photonuclear Q^2 range. FP64 inputs, FP32

@@ -236,3 +236,124 @@ def photo_transport_ratio(K: float, q: float, mass: float) -> float:
     f.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double]
     f.restype = ctypes.c_double
     return f(K, q, mass)
+
+
+# ---- PUMAS v1.2.1 DCS tabulation (oracle/pumas_oracle.hh) ----
+
+PUMAS_REF_PATH = os.path.join(HERE, "_ref", "libpumas_ref.so")
+_F = ctypes.POINTER(ctypes.c_float)
+_pumas_ref = None
+
+
+def _pumas_sigs(lib) -> None:
+    lib.oracle_pumas_dcs.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, _D, _D,
+                                     ctypes.c_int64, _D, ctypes.c_int]
+    lib.oracle_pumas_dcs.restype = ctypes.c_int
+    lib.oracle_pumas_n_table_dcs.argtypes = [ctypes.c_double]
+    lib.oracle_pumas_n_table_dcs.restype = ctypes.c_int
+    lib.oracle_pumas_dcs_table.argtypes = [ctypes.c_int, _D, _D, ctypes.c_double, ctypes.c_double, ctypes.c_int, _D,
+                                           _F, _F, _D, _F, _D, ctypes.c_int]
+    lib.oracle_pumas_dcs_table.restype = ctypes.c_int
+
+
+def pumas_dcs(process: int, Z: float, A: float, mass: float, K, q, threads: int = 1) -> np.ndarray:
+    """PUMAS model DCS (m^2/kg) of process 0, 1 or 2 over (K, q) pairs."""
+    lib = load()
+    _pumas_sigs(lib)
+    K, q = _prep(K, q)
+    out = np.zeros_like(K)
+    assert lib.oracle_pumas_dcs(process, float(Z), float(A), float(mass), _p(K), _p(q), K.size, _p(out),
+                                threads) == 0
+    return out
+
+
+def pumas_dilog(x: float) -> float:
+    """The oracle's real dilogarithm (pumas.c:12673-12738)."""
+    lib = load()
+    lib.oracle_pumas_dilog.argtypes = [ctypes.c_double]
+    lib.oracle_pumas_dilog.restype = ctypes.c_double
+    return lib.oracle_pumas_dilog(float(x))
+
+
+def pumas_rules():
+    """The oracle's 9- and 12-point Gauss-Legendre rows on [0, 1]."""
+    lib = load()
+    lib.oracle_pumas_rule.argtypes = [ctypes.c_int, _D, _D]
+    rows = []
+    for n in (9, 12):
+        x, w = np.zeros(n), np.zeros(n)
+        assert lib.oracle_pumas_rule(n, _p(x), _p(w)) == 0
+        rows.append((x, w))
+    return rows
+
+
+def pumas_n_table_dcs(cutoff: float) -> int:
+    lib = load()
+    _pumas_sigs(lib)
+    return lib.oracle_pumas_n_table_dcs(float(cutoff))
+
+
+def pumas_dcs_table(Z, A, mass: float, cutoff: float, K, threads: int = 1, fp64: bool = False):
+    """compute_dcs_table over elements (Z, A) and kinetic energies K:
+    (dcs [3, n_el, n_K, n_x, 2] float32, x [n_x] float32, x [n_x] float64,
+    envelope exponent [3, n_el, n_K] float32), plus the FP64 table of the
+    same layout when fp64."""
+    lib = load()
+    _pumas_sigs(lib)
+    Z = np.ascontiguousarray(Z, dtype=np.float64)
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    K = np.ascontiguousarray(K, dtype=np.float64)
+    n_x = lib.oracle_pumas_n_table_dcs(float(cutoff))
+    dcs = np.zeros((3, Z.size, K.size, n_x, 2), dtype=np.float32)
+    x = np.zeros(n_x, dtype=np.float32)
+    xd = np.zeros(n_x, dtype=np.float64)
+    env = np.zeros((3, Z.size, K.size, 2), dtype=np.float32)
+    dd = np.zeros(dcs.shape, dtype=np.float64) if fp64 else None
+    assert lib.oracle_pumas_dcs_table(Z.size, _p(Z), _p(A), float(mass), float(cutoff), K.size, _p(K),
+                                      dcs.ctypes.data_as(_F), x.ctypes.data_as(_F), _p(xd),
+                                      env.ctypes.data_as(_F), _p(dd) if fp64 else None, threads) == 0
+    return (dcs, x, xd, env[..., 0], dd) if fp64 else (dcs, x, xd, env[..., 0])
+
+
+def load_pumas_ref():
+    """The reference's vendored PUMAS compiled in place (oracle/_ref), or None."""
+    global _pumas_ref
+    if _pumas_ref is None and os.path.exists(PUMAS_REF_PATH):
+        lib = ctypes.CDLL(PUMAS_REF_PATH)
+        lib.ref_pumas_dcs.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                      ctypes.c_double, _D, _D, ctypes.c_long, _D]
+        lib.ref_pumas_dcs.restype = ctypes.c_int
+        lib.ref_pumas_n_table_dcs.argtypes = [ctypes.c_double]
+        lib.ref_pumas_n_table_dcs.restype = ctypes.c_int
+        lib.ref_pumas_dcs_table.argtypes = [ctypes.c_int, _D, _D, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                            _D, _F, _F, _F]
+        lib.ref_pumas_dcs_table.restype = ctypes.c_int
+        _pumas_ref = lib
+    return _pumas_ref
+
+
+def ref_pumas_dcs(process: int, Z: float, A: float, mass: float, K, q) -> np.ndarray:
+    ref = load_pumas_ref()
+    if ref is None:
+        raise FileNotFoundError(PUMAS_REF_PATH)
+    K, q = _prep(K, q)
+    out = np.zeros_like(K)
+    assert ref.ref_pumas_dcs(process, float(Z), float(A), 0.0, float(mass), _p(K), _p(q), K.size, _p(out)) == 0
+    return out
+
+
+def ref_pumas_dcs_table(Z, A, mass: float, cutoff: float, K):
+    """The reference's compute_dcs_table: (dcs, x, envelope [3, n_el, n_K, 2])."""
+    ref = load_pumas_ref()
+    if ref is None:
+        raise FileNotFoundError(PUMAS_REF_PATH)
+    Z = np.ascontiguousarray(Z, dtype=np.float64)
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    K = np.ascontiguousarray(K, dtype=np.float64)
+    n_x = ref.ref_pumas_n_table_dcs(float(cutoff))
+    dcs = np.zeros((3, Z.size, K.size, n_x, 2), dtype=np.float32)
+    x = np.zeros(n_x, dtype=np.float32)
+    env = np.zeros((3, Z.size, K.size, 2), dtype=np.float32)
+    assert ref.ref_pumas_dcs_table(Z.size, _p(Z), _p(A), float(mass), float(cutoff), K.size, _p(K),
+                                   dcs.ctypes.data_as(_F), x.ctypes.data_as(_F), env.ctypes.data_as(_F)) == 0
+    return dcs, x, env

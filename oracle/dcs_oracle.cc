@@ -15,6 +15,7 @@
 #include "coulomb_oracle.hh"
 #include "dcs_oracle.hh"
 #include "opcount.hh"
+#include "pumas_oracle.hh"
 
 namespace noa_oracle {
 thread_local OpTally g_tally;
@@ -166,6 +167,66 @@ int oracle_soft_scattering(const double* K, int64_t n, double A, double I, int32
 // The mu-integrated ratio at one photonuclear node (dcs.hh:910-932).
 double oracle_photo_transport_ratio(double K, double q, double mass) {
     return noa_oracle::photonuclear_transport_ratio(K, q, mass);
+}
+
+// PUMAS v1.2.1 DCS models through their m^2/kg wrappers (pumas.c:11067-11112),
+// processes 0 (SSR bremsstrahlung), 1 (SSR pair production), 2 (DRSS
+// photonuclear): out[i] = dcs(K[i], q[i]).
+int oracle_pumas_dcs(int process, double Z, double A, double mass, const double* K, const double* q, int64_t n,
+                     double* out, int threads) {
+    if (process < 0 || process > 2) return -1;
+#pragma omp parallel for num_threads(threads > 0 ? threads : omp_get_max_threads()) if (threads != 1)
+    for (int64_t i = 0; i < n; i++) out[i] = noa_oracle::pumas::dcs(process, Z, A, mass, K[i], q[i]);
+    return 0;
+}
+
+int oracle_pumas_n_table_dcs(double cutoff) { return noa_oracle::pumas::n_table_dcs(cutoff); }
+
+double oracle_pumas_dilog(double x) { return noa_oracle::pumas::dilog(x); }
+
+// The 9- (n = 9) or 12-point (n = 12) Gauss-Legendre row on [0, 1].
+int oracle_pumas_rule(int n, double* x, double* w) {
+    namespace pm = noa_oracle::pumas;
+    if (n != 9 && n != 12) return -1;
+    for (int i = 0; i < n; i++) {
+        x[i] = n == 9 ? pm::GQ9_X[i] : pm::GQ12_X[i];
+        w[i] = n == 9 ? pm::GQ9_W[i] : pm::GQ12_W[i];
+    }
+    return 0;
+}
+
+// compute_dcs_table (pumas.c:11693-11787) for n_el elements and n_K kinetic
+// energies, in PUMAS's layouts: dcs[(p n_el + e) n_K + row][2 n_x],
+// envelope[(p n_el + e) n_K + row][2] (the exponent; the maximum, entry 1,
+// is left 0), x[n_x] as floats; x_d[n_x] the same nodes in FP64; dcs_d,
+// when not NULL, the table in FP64 (same layout as dcs).
+int oracle_pumas_dcs_table(int n_el, const double* Z, const double* A, double mass, double cutoff, int n_K,
+                           const double* K, float* dcs, float* x, double* x_d, float* envelope, double* dcs_d,
+                           int threads) {
+    namespace pm = noa_oracle::pumas;
+    const int n = pm::n_table_dcs(cutoff);
+    pm::x_grid(cutoff, n, x_d);
+    for (int i = 0; i < n; i++) x[i] = float(x_d[i]);
+    const int64_t rows = int64_t(3) * n_el * n_K;
+#pragma omp parallel num_threads(threads > 0 ? threads : omp_get_max_threads()) if (threads != 1)
+    {
+        double* y = new double[n];
+        double* m = new double[n];
+#pragma omp for schedule(dynamic, 1)
+        for (int64_t r = 0; r < rows; r++) {
+            const int p = int(r / (int64_t(n_el) * n_K));
+            const int e = int((r / n_K) % n_el);
+            const int row = int(r % n_K);
+            pm::tabulate_row(p, Z[e], A[e], mass, cutoff, K[row], n, x_d, y, m, dcs + r * 2 * n, envelope + 2 * r,
+                             dcs_d ? dcs_d + r * 2 * n : nullptr);
+            envelope[2 * r + 1] = 0.f;
+        }
+        delete[] y;
+        delete[] m;
+    }
+    for (int p = 0; p < 3; p++)
+        for (int e = 0; e < n_el; e++) pm::fill_low_energy_alpha(n_K, envelope + 2 * (int64_t(p) * n_el + e) * n_K, 2);
+    return 0;
 }
 
 }  // extern "C"

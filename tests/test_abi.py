@@ -15,7 +15,7 @@ HEADER = os.path.join(ROOT, "include", "noa_dcs.h")
 
 def header_functions() -> set:
     text = open(HEADER).read()
-    return set(re.findall(r"^(?:int|int64_t|const char\*)\s+(noa_dcs_\w+)\s*\(", text, re.M))
+    return set(re.findall(r"^(?:int|int64_t|const char\*)\s+(noa_(?:dcs|pumas)_\w+)\s*\(", text, re.M))
 
 
 def test_header_declares_the_bound_signatures():
@@ -26,7 +26,7 @@ def test_library_exports_every_declared_symbol():
     lib = _lib.lib()
     assert _lib.missing_symbols() == []
     out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True, text=True).stdout
-    exported = set(re.findall(r"\sT\s(noa_dcs_\w+)", out))
+    exported = set(re.findall(r"\sT\s(noa_(?:dcs|pumas)_\w+)", out))
     assert header_functions() <= exported
     assert lib.noa_dcs_version().decode().startswith("noa_dcs")
 
