@@ -231,6 +231,35 @@ def consumers(dev, stream, steps: int) -> dict:
     return out
 
 
+def pumas_table_entry(dev, stream, steps: int, with_cpu: bool) -> dict:
+    """PUMAS v1.2.1 DCS tabulation (compute_dcs_table) for the 48-element
+    batch's elements at 146 kinetic energies 1e-3..1e6 GeV, muon, cutoff 0.05:
+    3 x 48 x 146 rows of 36 nodes. Rows per second on the GPU, and the
+    restated oracle (OpenMP) on the same rows."""
+    import numpy as np
+    import torch
+
+    from noa_amd import pumas, workload
+    from noa_amd.physics import MUON_MASS
+    elements, _ = workload.element_batch(48, seed=0)
+    Z = np.array([float(e.Z) for e in elements])
+    A = np.array([e.A for e in elements])
+    Kh = 10.0 ** np.linspace(-3, 6, 146)
+    K = torch.from_numpy(Kh).to(dev)
+    rows = 3 * Z.size * Kh.size
+    w, k = timed_launches(lambda: pumas.dcs_table(Z, A, MUON_MASS, 0.05, K), steps, 1, stream)
+    out = {"rows_per_s": rows * steps / w, "ms_per_table": k, "rows": rows, "nodes_per_row": pumas.n_table_dcs(0.05),
+           "config": "48 elements x 146 kinetic energies (1e-3..1e6 GeV) x 3 processes, muon, cutoff 0.05"}
+    if with_cpu:
+        import oracle
+        t0 = time.perf_counter()
+        oracle.pumas_dcs_table(Z, A, MUON_MASS, 0.05, Kh, threads=CPU_THREADS)
+        dt = time.perf_counter() - t0
+        out["cpu_oracle"] = {"rows_per_s": rows / dt, "cores": CPU_THREADS, "kind": "port",
+                             "sample": f"the same {rows} rows, OpenMP over rows, {dt:.2f} s"}
+    return out
+
+
 def main() -> None:
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -353,6 +382,7 @@ def main() -> None:
                                            "config": "48 dcs.eval_f64 calls from Python per batch (host "
                                                      "dispatch between the launches included)"}
         del outs_b
+        forms["pumas_dcs_table"] = pumas_table_entry(dev, stream, max(3, args.steps // 10), not args.no_cpu)
         result["forms"] = forms
 
     if args.calibrate_copy:
