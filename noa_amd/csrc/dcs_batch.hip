@@ -57,7 +57,7 @@ template <class P, int TILE>
 __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(BatchMinWaves<P>::value))) void dcs_batch_kernel(
     const double* __restrict__ K, const double* __restrict__ q, const DcsConsts* __restrict__ consts,
     const WorkItem* __restrict__ items, uint32_t n_items, uint32_t* counter, Outs<typename P::out_t> o) {
-    __shared__ TileLists<TILE> s;
+    __shared__ TileLists<TILE, P::has_redo> s;
     __shared__ uint32_t s_item;
     __shared__ __attribute__((aligned(fm::TABLE_LDS_ALIGN))) double fm_block[fm::TABLE_DOUBLES];
     load_fm_tables(fm_block);  // published by the barrier below

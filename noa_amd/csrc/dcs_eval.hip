@@ -18,7 +18,7 @@ constexpr int EVAL_TILE = 2048;
 template <class P, int TILE>
 __global__ __launch_bounds__(WG) void dcs_eval_kernel(const double* __restrict__ K, const double* __restrict__ q,
                                                       int64_t n, DcsConsts c, Outs<typename P::out_t> o) {
-    __shared__ TileLists<TILE> s;
+    __shared__ TileLists<TILE, P::has_redo> s;
     __shared__ __attribute__((aligned(fm::TABLE_LDS_ALIGN))) double fm_block[fm::TABLE_DOUBLES];
     load_fm_tables(fm_block);  // published by run_tile's first barrier
     const int64_t base = int64_t(blockIdx.x) * TILE;

@@ -102,15 +102,14 @@ struct F32MN {
 
     // The range the FP32 arithmetic is laid out for: 1e-20 <= K <= 1e12 GeV,
     // 1e-30 <= q <= 1e12 GeV and q >= 1e-30 E, where the FP32 products and
-    // quotients below (E^2, 1 / nu, q^2, ...) stay finite and normal. Pairs
-    // outside it (NaN fails the test) take the FP64 physics: the closed forms
-    // directly, the integrals through the redo list.
+    // quotients below (E^2, 1 / nu, q^2, ...) stay finite and normal. The
+    // tile sends pairs outside it (NaN fails the test) whole to the FP64
+    // physics, so the functions below see only pairs inside it.
     static __device__ __forceinline__ bool in_range(double K, double q, const DcsConsts& c) {
         return (K >= 1e-20) && (K <= 1e12) && (q >= 1e-30) && (q <= 1e12) && (q >= 1e-30 * (K + c.mass));
     }
 
     static __device__ __forceinline__ float brems(double K, double q, const DcsConsts& c, const fm::Tab& t) {
-        if (!in_range(K, q, c)) return float(F64::brems(K, q, c, t));
         const float me = float(ELECTRON_MASS);
         const float sqrte = float(SQRTE_BREMS);
         const double E = K + c.mass;
@@ -149,7 +148,6 @@ struct F32MN {
     }
 
     static __device__ __forceinline__ float ion(double K, double q, const DcsConsts& c, const fm::Tab& t) {
-        if (!in_range(K, q, c)) return float(F64::ion(K, q, c, t));
         const double me = ELECTRON_MASS;
         double P2, E, Wmax;
         if (!ion_open(K, q, c, P2, E, Wmax)) return 0.f;
@@ -176,11 +174,9 @@ struct F32MN {
     // edges, e.g. q -> 4 m_e) exceeds PAIR_REDO_TOL of the integral.
     static constexpr float PAIR_REDO_TOL = 2e-6f;
 
+    // (pair and photo see only pairs inside in_range: the tile sends the
+    // others whole to the FP64 physics)
     static __device__ float pair(double K, double q, const DcsConsts& c, bool& redo, const fm::Tab& t) {
-        if (!in_range(K, q, c)) {
-            redo = true;
-            return 0.f;
-        }
         // FP64 set-up: the integration bound exactly as dcs.hh:169-176
         double tmin_d;
         {
@@ -277,8 +273,6 @@ struct F32MN {
     }
 
     static __device__ float photo(double K, double q, const DcsConsts& c, bool& redo, const fm::Tab& t) {
-        redo = !in_range(K, q, c);
-        if (redo) return 0.f;
         const double M = PHOTO_M;
         const double mpi = PHOTO_MPI;
         const double E = K + c.mass;

@@ -11,9 +11,11 @@
 //      (wave ballot + one LDS atomic per wave);
 //   2. all lanes drain the two lists densely, heaviest process first, so
 //      every wave runs the integrals with (almost) all lanes active;
-//   3. (mixed precision only) pairs whose FP32 integral failed its own error
-//      estimate were appended to a third list in phase 2 and are recomputed
-//      densely with the FP64 physics.
+//   3. (mixed precision only) pairs outside the FP32 form's range were
+//      appended to a third list in phase 1 (all four processes), and pairs
+//      whose FP32 integral failed its own error estimate to the same list in
+//      phase 2 (that integral); they are recomputed densely with the FP64
+//      physics.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -36,18 +38,24 @@ struct Outs {
     OutT* i;
 };
 
-template <int TILE>
+// The tile's lists in LDS. A pair enters the redo list at most twice (once
+// for all four processes from phase 1, or once per failed integral from
+// phase 2), so a form with a redo list holds 2 TILE entries; the FP64 form
+// holds none.
+template <int TILE, bool REDO>
 struct TileLists {
-    static_assert(TILE <= 32768, "15-bit tile offsets (bit 15 tags the process in the redo list)");
+    static_assert(TILE <= 16384, "14-bit tile offsets (bits 14 and 15 tag the process in the redo list)");
     uint16_t photo[TILE];
     uint16_t pair[TILE];
-    uint16_t redo[TILE];
+    uint16_t redo[REDO ? 2 * TILE : 1];
     uint32_t n_photo;
     uint32_t n_pair;
     uint32_t n_redo;
 };
 
-constexpr uint32_t REDO_PHOTO = 0x8000u;
+constexpr uint32_t REDO_PHOTO = 0x8000u;  // the photonuclear integral of the pair
+constexpr uint32_t REDO_ALL = 0x4000u;    // all four processes of the pair
+constexpr uint32_t REDO_OFFSET = 0x3FFFu;
 
 // Appends `value` for every active lane with pred set; one LDS atomic per
 // wave. Lanes outside the current EXEC mask take no part, so it may be called
@@ -85,7 +93,7 @@ __device__ __forceinline__ void load_fm_tables(double* block) {
 template <class P, int TILE>
 __device__ __forceinline__ void run_tile(int64_t base, int64_t end, const double* __restrict__ K,
                                          const double* __restrict__ q, const DcsConsts& c,
-                                         const Outs<typename P::out_t>& o, TileLists<TILE>& s,
+                                         const Outs<typename P::out_t>& o, TileLists<TILE, P::has_redo>& s,
                                          const fm::Tab& t) {
     using out_t = typename P::out_t;
     constexpr int ITEMS = TILE / WG;
@@ -116,8 +124,10 @@ __device__ __forceinline__ void run_tile(int64_t base, int64_t end, const double
             k_next = K[gi + WG];
             q_next = q[gi + WG];
         }
-        bool pa = false, pha = false;
-        if (valid) {
+        bool pa = false, pha = false, full_redo = false;
+        // (mixed form) a pair outside the FP32 range goes whole to phase 3
+        if constexpr (P::has_redo) full_redo = valid && !P::in_range(k, qq, c);
+        if (valid && !full_redo) {
             if (o.b) store_stream(&o.b[gi], P::brems(k, qq, c, t));
             if (o.i) store_stream(&o.i[gi], P::ion(k, qq, c, t));
             if (want_pair) {
@@ -131,6 +141,7 @@ __device__ __forceinline__ void run_tile(int64_t base, int64_t end, const double
         }
         if (want_photo) wave_append(pha, &s.n_photo, s.photo, uint32_t(local));
         if (want_pair) wave_append(pa, &s.n_pair, s.pair, uint32_t(local));
+        if constexpr (P::has_redo) wave_append(full_redo, &s.n_redo, s.redo, uint32_t(local) | REDO_ALL);
     }
     __syncthreads();
 
@@ -161,11 +172,18 @@ __device__ __forceinline__ void run_tile(int64_t base, int64_t end, const double
 #pragma unroll 1
         for (uint32_t v = tid; v < nr; v += WG) {
             const uint32_t tag = s.redo[v];
-            const int64_t g = base + (tag & (REDO_PHOTO - 1));
-            if (tag & REDO_PHOTO)
+            const int64_t g = base + (tag & REDO_OFFSET);
+            if (tag & REDO_ALL) {
+                const double k = K[g], qq = q[g];
+                if (o.b) o.b[g] = out_t(F64::brems(k, qq, c, t));
+                if (o.i) o.i[g] = out_t(F64::ion(k, qq, c, t));
+                if (o.p) o.p[g] = out_t(F64::pair_active(k, qq, c) ? F64::pair_ieee(k, qq, c, t) : 0.);
+                if (o.ph) o.ph[g] = out_t(F64::photo_active(k, qq, c) ? F64::photo_ieee(k, qq, c, t) : 0.);
+            } else if (tag & REDO_PHOTO) {
                 o.ph[g] = out_t(F64::photo_ieee(K[g], q[g], c, t));
-            else
+            } else {
                 o.p[g] = out_t(F64::pair_ieee(K[g], q[g], c, t));
+            }
         }
     }
     __syncthreads();  // the lists are reused by the next tile of a persistent workgroup

@@ -133,3 +133,39 @@ def test_batch48_at_real_counts(precision, tol, device):
             c = assert_parity(got[p][a:b], exp[p], tol, f"{precision} element {e} (Z={el.Z}) process {p}")
             worst = max(worst, c["max_rel"])
     print(f"{precision}: worst max rel {worst:.3e} over {nb} pairs", flush=True)
+
+
+@pytest.mark.parametrize("kind", ["nan", "huge", "mixed"])
+def test_mixed_full_tiles_outside_range(kind, device):
+    """Whole tiles of pairs outside the FP32 form's range, where every pair
+    enters the FP64 redo list for its closed forms and, with both integrals
+    open, for both integrals too (three entries per pair): the mixed form
+    gives the oracle's values, NaNs included, through the single launch and
+    the batched launch."""
+    n = 3 * 2048 + 77
+    rng = np.random.default_rng(9)
+    if kind == "nan":
+        K = np.full(n, np.nan)
+        q = np.full(n, np.nan)
+    elif kind == "huge":
+        K = 10.0 ** rng.uniform(13, 30, n)
+        q = K * 10.0 ** rng.uniform(-3, -0.01, n)
+    else:
+        K = 10.0 ** rng.uniform(-3, 30, n)
+        q = K * 10.0 ** rng.uniform(-8, -0.001, n)
+        K[::5] = np.nan
+    with np.errstate(all="ignore"):
+        exp = oracle.dcs_map_all(K, q, STANDARD_ROCK, MUON_MASS, threads=0)
+    print(f"mixed form on {n} {kind} pairs", flush=True)
+    got = _gpu(K, q, STANDARD_ROCK, MUON_MASS, device, mixed=True)
+    for p in range(4):
+        _assert_mixed(got[p], exp[p], f"{kind} mixed process {p}")
+    Kd = torch.from_numpy(K).to(device)
+    qd = torch.from_numpy(q).to(device)
+    plan = BatchPlan([STANDARD_ROCK], np.array([0, n]), Kd, qd, MUON_MASS, precision="f32mixed", tile=4096,
+                     waves_per_cu=8)
+    plan.launch()
+    torch.cuda.synchronize()
+    for p in range(4):
+        _assert_mixed(plan.outputs[p].double().cpu().numpy(), exp[p], f"{kind} batched mixed process {p}")
+    plan.close()
