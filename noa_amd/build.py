@@ -20,7 +20,7 @@ LIB = os.path.join(HERE, "libnoa_dcs.so")
 BUILD_DIR = os.path.join(HERE, "_build")
 
 HIP_SOURCES = ["dcs_eval.hip", "dcs_batch.hip", "dcs_recoil.hip", "dcs_coulomb.hip", "surrogate_train.hip",
-               "surrogate_infer.hip", "pumas_table.hip"]
+               "surrogate_step.hip", "surrogate_infer.hip", "pumas_table.hip"]
 
 # host-compiled sources of the library (the CPU-tensor path), built with g++
 HOST_SOURCES = ["dcs_host.cc"]

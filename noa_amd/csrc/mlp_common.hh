@@ -66,5 +66,22 @@ constexpr int FRAG_HIDDEN = 3 * 4 * 8 * 64 * 8;
 constexpr int FRAG_OUT = 8 * 64 * 8;
 constexpr int FRAG_TOTAL = FRAG_HIDDEN + FRAG_OUT;
 
+// The transposed weights for back-propagation, W^T as the A operand: rows are
+// a layer's input features (natural order, 4 M-tiles of 32), K its output
+// features in the frag_k order, so that the gradient of a layer's output, held
+// as that layer's forward output, is the B operand as it stands. Hidden
+// layers as FRAG_HIDDEN; the output layer's 4 outputs fill K-block 0
+// (slots h = 0, e < 4), 4 M-tiles x 64 lanes x 8.
+constexpr int FRAGT_HIDDEN = 3 * 4 * 8 * 64 * 8;
+constexpr int FRAGT_OUT = 4 * 64 * 8;
+constexpr int FRAGT_TOTAL = FRAGT_HIDDEN + FRAGT_OUT;
+
+// Inverse of frag_k: the (K-block, lane half, element) slot of logical K index k.
+__host__ __device__ constexpr int slot_b(int k) { return 2 * (k / 32) + (k % 32) / 16; }
+__host__ __device__ constexpr int slot_h(int k) { return (k % 8) / 4; }
+__host__ __device__ constexpr int slot_e(int k) { return 4 * ((k % 16) / 8) + k % 4; }
+static_assert(frag_k(slot_b(77), slot_h(77), slot_e(77)) == 77, "slot inverse");
+static_assert(frag_k(slot_b(126), slot_h(126), slot_e(126)) == 126, "slot inverse");
+
 }  // namespace mlp
 }  // namespace noa_hip

@@ -49,23 +49,6 @@ struct MlpLds {
     float b1[H];
 };
 
-// Packs layers 2-5 into MFMA A-operand fragments (see frag_k).
-__global__ void pack_kernel(const float* __restrict__ P, uint16_t* __restrict__ frag) {
-    for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < FRAG_TOTAL; idx += gridDim.x * blockDim.x) {
-        float v;
-        if (idx < FRAG_HIDDEN) {
-            const int e = idx % 8, lane = (idx / 8) % 64, b = (idx / 512) % 8, m = (idx / 4096) % 4, li = idx / 16384;
-            v = P[HIDDEN_W[li] + (32 * m + lane % 32) * H + frag_k(b, lane / 32, e)];
-        } else {
-            const int j = idx - FRAG_HIDDEN;
-            const int e = j % 8, lane = (j / 8) % 64, b = j / 512;
-            const int row = lane % 32;
-            v = row < OUT ? P[OFF_W5 + row * H + frag_k(b, lane / 32, e)] : 0.f;
-        }
-        frag[idx] = __builtin_bit_cast(uint16_t, __bf16(v));
-    }
-}
-
 __device__ __forceinline__ void load_lds(MlpLds& L, const uint16_t* __restrict__ frag, const float* __restrict__ P) {
     const uint4* src = reinterpret_cast<const uint4*>(frag);
     uint4* dst = reinterpret_cast<uint4*>(L.frag);
@@ -427,14 +410,6 @@ using namespace noa_hip;
 using namespace noa_hip::mlp;
 
 namespace {
-
-int ensure_packed(noa_dcs_mlp* net, hipStream_t s) {
-    if (net->packed_valid) return NOA_DCS_OK;
-    hipLaunchKernelGGL(pack_kernel, dim3((FRAG_TOTAL + 255) / 256), dim3(256), 0, s, net->params, net->packed);
-    if (int rc = check_hip(hipGetLastError(), "pack_kernel")) return rc;
-    net->packed_valid = true;
-    return NOA_DCS_OK;
-}
 
 int n_cus() {
     int dev = 0, n = 0;

@@ -1,14 +1,13 @@
-// Training of the DCS surrogate network (see mlp_common.hh) with hand-written
-// kernels: a bf16 MFMA GEMM (FP32 in/out, FP32 accumulation, split-K for the
-// weight gradients), the FP32 first layer, bias + SiLU forward/backward, the
-// masked squared-error loss and Adam. The host loop runs minibatch epochs
-// over a device-resident dataset.
+// Training of the DCS surrogate network (see mlp_common.hh): the network
+// object, the features, the layered forward (a bf16 MFMA GEMM with FP32
+// accumulation, the FP32 first layer, bias + SiLU; used to check the fused
+// kernels), and the host loop of minibatch epochs over a device-resident
+// dataset, whose steps are the three fused kernels of surrogate_step.hip.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
-#include <numeric>
 #include <random>
 #include <vector>
 
@@ -81,15 +80,6 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(int M, int N, int K, con
     }
 }
 
-// C[i] = sum_z P[z][i] for i < n.
-__global__ void splitk_reduce_kernel(const float* __restrict__ P, float* __restrict__ C, int n, int splits) {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        float s = 0.f;
-        for (int z = 0; z < splits; z++) s += P[int64_t(z) * n + i];
-        C[i] = s;
-    }
-}
-
 // First layer in FP32: Z1[r][f] = W1[f] . X[r] + b1[f]; H1 = silu(Z1).
 __global__ void layer1_fwd_kernel(const float* __restrict__ X, const float* __restrict__ P, float* __restrict__ Z1,
                                   float* __restrict__ H1, int rows) {
@@ -104,32 +94,6 @@ __global__ void layer1_fwd_kernel(const float* __restrict__ X, const float* __re
     }
 }
 
-// dW1[f][j] = sum_r dZ1[r][f] X[r][j], db1[f] = sum_r dZ1[r][f]; one block per f.
-__global__ __launch_bounds__(256) void layer1_grad_kernel(const float* __restrict__ X, const float* __restrict__ dZ1,
-                                                          float* __restrict__ G, int rows) {
-    const int f = blockIdx.x;
-    float s[4] = {0.f, 0.f, 0.f, 0.f};
-    for (int r = threadIdx.x; r < rows; r += blockDim.x) {
-        const float d = dZ1[int64_t(r) * H + f];
-        s[0] += d * X[int64_t(r) * IN];
-        s[1] += d * X[int64_t(r) * IN + 1];
-        s[2] += d * X[int64_t(r) * IN + 2];
-        s[3] += d;
-    }
-    __shared__ float red[4][256];
-    for (int j = 0; j < 4; j++) red[j][threadIdx.x] = s[j];
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (int(threadIdx.x) < w)
-            for (int j = 0; j < 4; j++) red[j][threadIdx.x] += red[j][threadIdx.x + w];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        for (int j = 0; j < 3; j++) G[OFF_W1 + f * IN + j] = red[j][0];
-        G[OFF_B1 + f] = red[3][0];
-    }
-}
-
 // Z += b (broadcast over rows); H = silu(Z) when act, else H = Z.
 __global__ void bias_act_kernel(float* __restrict__ Z, const float* __restrict__ b, float* __restrict__ Hout,
                                 int rows, int cols, int act) {
@@ -141,87 +105,12 @@ __global__ void bias_act_kernel(float* __restrict__ Z, const float* __restrict__
     }
 }
 
-// dZ = dH * silu'(Z), silu'(z) = s (1 + z (1 - s)), s = sigmoid(z).
-__global__ void silu_bwd_kernel(const float* __restrict__ dH, const float* __restrict__ Z, float* __restrict__ dZ,
-                                int64_t n) {
-    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
-        const float z = Z[i];
-        const float s = 1.0f / (1.0f + expf(-z));
-        dZ[i] = dH[i] * s * (1.0f + z * (1.0f - s));
-    }
-}
-
-// db[c] = sum_r dZ[r][c]; one block per column.
-__global__ __launch_bounds__(256) void bias_grad_kernel(const float* __restrict__ dZ, float* __restrict__ db, int rows,
-                                                        int cols) {
-    const int c = blockIdx.x;
-    float s = 0.f;
-    for (int r = threadIdx.x; r < rows; r += blockDim.x) s += dZ[int64_t(r) * cols + c];
-    __shared__ float red[256];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (int(threadIdx.x) < w) red[threadIdx.x] += red[threadIdx.x + w];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) db[c] = red[0];
-}
-
-// Masked squared error, normalised by rows * OUT: dY = 2 m (Y - T) / (rows OUT);
-// per-block partial sums of m (Y - T)^2 / (rows OUT) go to partial[blockIdx.x].
-__global__ __launch_bounds__(256) void loss_kernel(const float* __restrict__ Y, const float* __restrict__ T,
-                                                   const float* __restrict__ Mk, float* __restrict__ dY,
-                                                   float* __restrict__ partial, int rows) {
-    const int n = rows * OUT;
-    const float inv = 1.0f / float(n);
-    float s = 0.f;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const float d = (Y[i] - T[i]) * Mk[i];
-        dY[i] = 2.0f * d * inv;
-        s += d * d * inv;
-    }
-    __shared__ float red[256];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (int(threadIdx.x) < w) red[threadIdx.x] += red[threadIdx.x + w];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
-}
-
-__global__ void adam_kernel(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
-                            float* __restrict__ v, int n, float lr, float b1, float b2, float eps, float bc1, float bc2) {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const float gi = g[i];
-        const float mi = b1 * m[i] + (1.f - b1) * gi;
-        const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
-        m[i] = mi;
-        v[i] = vi;
-        p[i] -= lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
-    }
-}
-
 __global__ void features_kernel(const double* __restrict__ K, const double* __restrict__ q, int64_t n, int Z,
                                 float* __restrict__ X) {
     for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
         float x[IN];
         features(K[i], q[i], Z, x);
         for (int j = 0; j < IN; j++) X[i * IN + j] = x[j];
-    }
-}
-
-// Rows idx[i] of X (3 wide), T and M (4 wide) gathered into a contiguous batch.
-__global__ void gather_kernel(const float* __restrict__ X, const float* __restrict__ T, const float* __restrict__ Mk,
-                              const int32_t* __restrict__ idx, float* __restrict__ Xb, float* __restrict__ Tb,
-                              float* __restrict__ Mb, int rows) {
-    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < rows; r += gridDim.x * blockDim.x) {
-        const int64_t s = idx[r];
-        for (int j = 0; j < IN; j++) Xb[r * IN + j] = X[s * IN + j];
-        for (int j = 0; j < OUT; j++) {
-            Tb[r * OUT + j] = T[s * OUT + j];
-            Mb[r * OUT + j] = Mk[s * OUT + j];
-        }
     }
 }
 
@@ -241,18 +130,10 @@ inline unsigned grid_for(int64_t n, int64_t per_block = 256, int64_t cap = 4096)
 }
 
 template <bool TA, bool TB>
-hipError_t gemm(hipStream_t s, int M, int N, int K, const float* A, int lda, const float* B, int ldb, float* C, int ldc,
-                float* ws, int splits) {
-    if (splits <= 1) {
-        hipLaunchKernelGGL((gemm_bf16_kernel<TA, TB>), dim3((N + GT - 1) / GT, (M + GT - 1) / GT, 1), dim3(256), 0, s,
-                           M, N, K, A, lda, B, ldb, C, ldc, K);
-        return hipGetLastError();
-    }
-    const int chunk = ((K + splits - 1) / splits + GK - 1) / GK * GK;
-    const int sp = (K + chunk - 1) / chunk;
-    hipLaunchKernelGGL((gemm_bf16_kernel<TA, TB>), dim3((N + GT - 1) / GT, (M + GT - 1) / GT, sp), dim3(256), 0, s, M,
-                       N, K, A, lda, B, ldb, ws, N, chunk);
-    hipLaunchKernelGGL(splitk_reduce_kernel, dim3(grid_for(int64_t(M) * N)), dim3(256), 0, s, ws, C, M * N, sp);
+hipError_t gemm(hipStream_t s, int M, int N, int K, const float* A, int lda, const float* B, int ldb, float* C,
+                int ldc) {
+    hipLaunchKernelGGL((gemm_bf16_kernel<TA, TB>), dim3((N + GT - 1) / GT, (M + GT - 1) / GT, 1), dim3(256), 0, s, M,
+                       N, K, A, lda, B, ldb, C, ldc, K);
     return hipGetLastError();
 }
 
@@ -263,58 +144,16 @@ hipError_t forward(noa_dcs_mlp* net, const float* X, int rows, hipStream_t s) {
     hipLaunchKernelGGL(layer1_fwd_kernel, dim3(grid_for(int64_t(rows) * H)), dim3(256), 0, s, X, P, w.Z[0], w.A[0],
                        rows);
     for (int l = 0; l < 3; l++) {
-        hipError_t e = gemm<false, true>(s, rows, H, H, w.A[l], H, P + HIDDEN_W[l], H, w.Z[l + 1], H, nullptr, 1);
+        hipError_t e = gemm<false, true>(s, rows, H, H, w.A[l], H, P + HIDDEN_W[l], H, w.Z[l + 1], H);
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(bias_act_kernel, dim3(grid_for(int64_t(rows) * H)), dim3(256), 0, s, w.Z[l + 1],
                            P + HIDDEN_B[l], w.A[l + 1], rows, H, 1);
     }
-    hipError_t e = gemm<false, true>(s, rows, OUT, H, w.A[3], H, P + OFF_W5, H, w.Y, OUT, nullptr, 1);
+    hipError_t e = gemm<false, true>(s, rows, OUT, H, w.A[3], H, P + OFF_W5, H, w.Y, OUT);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(bias_act_kernel, dim3(grid_for(int64_t(rows) * OUT)), dim3(256), 0, s, w.Y, P + OFF_B5, w.Y,
                        rows, OUT, 0);
     return hipGetLastError();
-}
-
-// Backward from dY = dLoss/dY (in w.dY) into net->grads.
-hipError_t backward(noa_dcs_mlp* net, const float* X, int rows, hipStream_t s) {
-    const float* P = net->params;
-    float* G = net->grads;
-    MlpWork& w = net->work;
-    const int splits = std::max(1, std::min(64, rows / 256));
-    hipError_t e = gemm<true, false>(s, OUT, H, rows, w.dY, OUT, w.A[3], H, G + OFF_W5, H, w.splitk, splits);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(bias_grad_kernel, dim3(OUT), dim3(256), 0, s, w.dY, G + OFF_B5, rows, OUT);
-    // dA4 = dY W5
-    e = gemm<false, false>(s, rows, H, OUT, w.dY, OUT, P + OFF_W5, H, w.dA, H, nullptr, 1);
-    if (e != hipSuccess) return e;
-    for (int l = 2; l >= 0; l--) {  // hidden layer l + 2: Z[l+1] = A[l] W^T + b
-        hipLaunchKernelGGL(silu_bwd_kernel, dim3(grid_for(int64_t(rows) * H)), dim3(256), 0, s, w.dA, w.Z[l + 1], w.dZ,
-                           int64_t(rows) * H);
-        e = gemm<true, false>(s, H, H, rows, w.dZ, H, w.A[l], H, G + HIDDEN_W[l], H, w.splitk, splits);
-        if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(bias_grad_kernel, dim3(H), dim3(256), 0, s, w.dZ, G + HIDDEN_B[l], rows, H);
-        e = gemm<false, false>(s, rows, H, H, w.dZ, H, P + HIDDEN_W[l], H, w.dA, H, nullptr, 1);
-        if (e != hipSuccess) return e;
-    }
-    hipLaunchKernelGGL(silu_bwd_kernel, dim3(grid_for(int64_t(rows) * H)), dim3(256), 0, s, w.dA, w.Z[0], w.dZ,
-                       int64_t(rows) * H);
-    hipLaunchKernelGGL(layer1_grad_kernel, dim3(H), dim3(256), 0, s, X, w.dZ, G, rows);
-    return hipGetLastError();
-}
-
-hipError_t loss_and_grad(noa_dcs_mlp* net, const float* T, const float* Mk, int rows, hipStream_t s) {
-    MlpWork& w = net->work;
-    hipLaunchKernelGGL(loss_kernel, dim3(LOSS_BLOCKS), dim3(256), 0, s, w.Y, T, Mk, w.dY, w.loss_partial, rows);
-    return hipGetLastError();
-}
-
-double read_loss(noa_dcs_mlp* net, hipStream_t s) {
-    float part[LOSS_BLOCKS];
-    (void)hipMemcpyAsync(part, net->work.loss_partial, sizeof(part), hipMemcpyDeviceToHost, s);
-    (void)hipStreamSynchronize(s);
-    double acc = 0;
-    for (float p : part) acc += p;
-    return acc;
 }
 
 }  // namespace
@@ -401,15 +240,14 @@ int noa_dcs_mlp_forward(noa_dcs_mlp* net, const float* X, int64_t rows, float* Y
     return ok();
 }
 
-// One forward + backward on rows <= max_rows; loss and gradients to the host.
+// One forward + backward on rows <= max_rows (the fused step kernels, no
+// update); loss and gradients to the host.
 int noa_dcs_mlp_forward_backward(noa_dcs_mlp* net, const float* X, const float* T, const float* Mk, int32_t rows,
                                  double* loss, float* grads_host, hipStream_t s) {
     if (!net || !X || !T || !Mk) return fail(NOA_DCS_EINVAL, "NULL argument");
     if (rows < 1 || rows > net->max_rows) return fail(NOA_DCS_ERANGE, "rows must be in [1, max_rows]");
-    if (int rc = check_hip(forward(net, X, rows, s), "mlp forward")) return rc;
-    if (int rc = check_hip(loss_and_grad(net, T, Mk, rows, s), "mlp loss")) return rc;
-    if (int rc = check_hip(backward(net, X, rows, s), "mlp backward")) return rc;
-    if (loss) *loss = read_loss(net, s);
+    if (int rc = train_step(net, X, T, Mk, RowOrder{}, 0, rows, false, 0.f, 1.f, 1.f, nullptr, s)) return rc;
+    if (loss) *loss = step_loss(net, s);
     if (grads_host)
         if (int rc = check_hip(hipMemcpy(grads_host, net->grads, NPARAM * sizeof(float), hipMemcpyDeviceToHost),
                                "hipMemcpy(grads)"))
@@ -418,54 +256,43 @@ int noa_dcs_mlp_forward_backward(noa_dcs_mlp* net, const float* X, const float* 
 }
 
 // Minibatch Adam over the device dataset (X [n x 3], T and M [n x 4]) for
-// `epochs` epochs; cosine learning rate from lr_max to lr_min. Mean loss per
-// epoch to loss_per_epoch (host, may be NULL).
+// `epochs` epochs; cosine learning rate from lr_max to lr_min. Each epoch
+// visits n / batch batches in its own keyed order (RowOrder); the mean loss
+// of each epoch's steps to loss_per_epoch (host, may be NULL), summed on the
+// device, so that the loop does not synchronise until it ends.
 int noa_dcs_mlp_train(noa_dcs_mlp* net, const float* X, const float* T, const float* Mk, int64_t n, int32_t batch,
                       int32_t epochs, float lr_max, float lr_min, uint64_t seed, float* loss_per_epoch, hipStream_t s) {
     if (!net || !X || !T || !Mk) return fail(NOA_DCS_EINVAL, "NULL argument");
     if (n < batch || batch < 1 || batch > net->max_rows) return fail(NOA_DCS_ERANGE, "need 1 <= batch <= min(n, max_rows)");
     if (n > (int64_t(1) << 30)) return fail(NOA_DCS_ERANGE, "n too large");
     if (epochs < 0) return fail(NOA_DCS_EINVAL, "epochs must be >= 0");
-    MlpWork& w = net->work;
-    std::vector<int32_t> perm(n);
-    std::iota(perm.begin(), perm.end(), 0);
-    std::mt19937_64 gen(seed);
+    if (epochs == 0) return ok();
     const int64_t steps_per_epoch = n / batch;
     const int64_t total_steps = steps_per_epoch * epochs;
-    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-    int32_t* d_perm = nullptr;
-    if (int rc = check_hip(hipMalloc(&d_perm, n * sizeof(int32_t)), "hipMalloc(perm)")) return rc;
-    int rc = NOA_DCS_OK;
+    const float b1 = 0.9f, b2 = 0.999f;
+    double* d_loss = nullptr;
+    if (int rc = check_hip(hipMalloc(&d_loss, size_t(epochs) * sizeof(double)), "hipMalloc(loss)")) return rc;
+    int rc = check_hip(hipMemsetAsync(d_loss, 0, size_t(epochs) * sizeof(double), s), "hipMemsetAsync(loss)");
     for (int ep = 0; ep < epochs && !rc; ep++) {
-        std::shuffle(perm.begin(), perm.end(), gen);
-        rc = check_hip(hipMemcpyAsync(d_perm, perm.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, s), "hipMemcpy");
-        double ep_loss = 0;
-        int64_t sampled = 0;
+        const RowOrder order = epoch_order(seed, ep, n);
         for (int64_t st = 0; st < steps_per_epoch && !rc; st++) {
             const int64_t step = int64_t(ep) * steps_per_epoch + st;
-            hipLaunchKernelGGL(gather_kernel, dim3(grid_for(batch)), dim3(256), 0, s, X, T, Mk, d_perm + st * batch, w.Xb,
-                               w.Tb, w.Mb, batch);
-            rc = check_hip(forward(net, w.Xb, batch, s), "mlp forward");
-            if (!rc) rc = check_hip(loss_and_grad(net, w.Tb, w.Mb, batch, s), "mlp loss");
-            if (!rc) rc = check_hip(backward(net, w.Xb, batch, s), "mlp backward");
             const float cosf_ = 0.5f * (1.f + std::cos(float(M_PI) * float(step) / float(std::max<int64_t>(1, total_steps))));
             const float lr = lr_min + (lr_max - lr_min) * cosf_;
             net->adam_t++;
             const float bc1 = 1.f - std::pow(b1, float(net->adam_t));
             const float bc2 = 1.f - std::pow(b2, float(net->adam_t));
-            hipLaunchKernelGGL(adam_kernel, dim3(grid_for(NPARAM)), dim3(256), 0, s, net->params, net->grads, net->adam_m,
-                               net->adam_v, int(NPARAM), lr, b1, b2, eps, bc1, bc2);
-            if (!rc) rc = check_hip(hipGetLastError(), "adam");
-            if (loss_per_epoch && st % 16 == 0 && !rc) {  // sampled, each sample a synchronisation
-                ep_loss += read_loss(net, s);
-                sampled++;
-            }
+            rc = train_step(net, X, T, Mk, order, st * batch, batch, true, lr, bc1, bc2, d_loss + ep, s);
         }
-        if (loss_per_epoch) loss_per_epoch[ep] = float(ep_loss / double(std::max<int64_t>(1, sampled)));
     }
-    net->packed_valid = false;
-    (void)hipStreamSynchronize(s);
-    (void)hipFree(d_perm);
+    std::vector<double> ep_loss(epochs, 0.);
+    if (!rc)
+        rc = check_hip(hipMemcpyAsync(ep_loss.data(), d_loss, size_t(epochs) * sizeof(double), hipMemcpyDeviceToHost, s),
+                       "hipMemcpy(loss)");
+    if (!rc) rc = check_hip(hipStreamSynchronize(s), "hipStreamSynchronize");
+    (void)hipFree(d_loss);
+    if (!rc && loss_per_epoch)
+        for (int ep = 0; ep < epochs; ep++) loss_per_epoch[ep] = float(ep_loss[ep] / double(steps_per_epoch));
     return rc ? rc : ok();
 }
 

@@ -1,9 +1,11 @@
 """CPU FP64 reference of the surrogate network (test infrastructure): the
 same parameter layout and loss as noa_amd/csrc/mlp_common.hh and
 surrogate_train.hip, in numpy float64. With `bf16=True` the operands of the
-MFMA layers (activations, weights and back-propagated gradients) are rounded
-to bf16 exactly where the GPU rounds them, so the GPU's gradients can be
-compared tightly; with `bf16=False` it is the exact network."""
+MFMA layers (activations, weights and back-propagated gradients, and both
+operands of the hidden weight gradients) are rounded to bf16 exactly where
+the GPU rounds them (the biases, W1 and W5 gradients are FP32 there), so the
+GPU's gradients can be compared tightly; with `bf16=False` it is the exact
+network."""
 from __future__ import annotations
 
 import numpy as np
@@ -69,7 +71,7 @@ def loss_and_grad(p: np.ndarray, X, T, M, round_bf16: bool = False):
     loss = float(np.sum(d * d) / (n * OUT))
     dY = 2.0 * d / (n * OUT)
     g = {}
-    g["W5"] = q(dY).T @ q(A[3])
+    g["W5"] = dY.T @ A[3]  # FP32 on the GPU
     g["b5"] = dY.sum(0)
     dA = q(dY) @ q(w["W5"])
     for l, zi in ((4, 3), (3, 2), (2, 1)):
