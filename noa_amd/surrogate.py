@@ -147,9 +147,9 @@ def trend_basis(X: torch.Tensor) -> torch.Tensor:
                         z * z], 1)
 
 
-def targets(dcs, elements, offsets, X, stats=None):
+def targets(dcs, elements, offsets, X, stats=None, exclude=None):
     """Standardised residual of ln(A * DCS) over a per-process cubic trend in
-    the features, mask = DCS > 0. The trend (least squares over the active
+    the features, mask = DCS > 0 and not excluded. The trend (least squares over the active
     pairs, FP64) carries the bulk of the dynamic range, which the FP32 epilogue
     of the inference kernel adds back, so that the BF16 network resolves only
     the residual (std 0.1-0.7 instead of 2.5-9 in ln units)."""
@@ -161,8 +161,10 @@ def targets(dcs, elements, offsets, X, stats=None):
     M = torch.zeros(n, NPROC, dtype=torch.float32, device=lnA.device)
     for p in range(NPROC):
         act = dcs[p] > 0
-        M[:, p] = act.float()
         Y[act, p] = torch.log(dcs[p][act]) + lnA[act]
+        if exclude is not None:  # [n x 4] bool: samples left out of the fit
+            act = act & ~exclude[:, p]
+        M[:, p] = act.float()
     Phi = trend_basis(X)
     if stats is None:
         mean, std, trend = np.zeros(NPROC), np.ones(NPROC), np.zeros((NPROC, NTREND))
@@ -307,14 +309,7 @@ def error_map(sur_out, exact, K, q, offsets, bins: int = MAP_BINS) -> torch.Tens
     flags send both to the exact path (free for the queries where the
     process is kinematically closed, which return 0 either way)."""
     dev = K.device
-    n = K.numel()
-    el = torch.empty(n, dtype=torch.int64, device=dev)
-    for e in range(len(offsets) - 1):
-        el[int(offsets[e]):int(offsets[e + 1])] = e
-    nu = q / K
-    ik = ((torch.log10(K) + 3.0) / 9.0 * bins).long().clamp(0, bins - 1)
-    inu = ((torch.log10(nu) + 6.0) / 6.0 * bins).long().clamp(0, bins - 1)
-    cell = (el * bins + ik) * bins + inu
+    cell = cell_index(K, q, offsets, bins)
     maps = []
     for p in range(NPROC):
         s = sur_out[p].double()
@@ -327,6 +322,98 @@ def error_map(sur_out, exact, K, q, offsets, bins: int = MAP_BINS) -> torch.Tens
                         torch.where(cnt < MIN_CELL_SAMPLES, torch.full_like(m, float("inf")), m))
         maps.append(m.reshape(len(offsets) - 1, bins, bins))
     return torch.stack(maps)
+
+
+LATTICE = 32  # lattice intervals per cell side in lattice_error_map
+
+
+def _lattice(bins: int, sub: int, dev):
+    """The closed lattice of (bins sub + 1)^2 (K, q) points over the domain
+    (log10 K in [-3, 6], log10 nu in [-6, 0]), K-major; nu = 1 is outside the
+    domain (q < K), so the last line sits just below it."""
+    side = bins * sub + 1
+    lk = torch.linspace(-3.0, 6.0, side, dtype=torch.float64, device=dev)
+    lnu = torch.linspace(-6.0, 0.0, side, dtype=torch.float64, device=dev)
+    Kg = (10.0 ** lk)[:, None].expand(side, side)
+    nug = (10.0 ** lnu)[None, :].expand(side, side)
+    nug = torch.where(nug >= 1.0, torch.full_like(nug, 1.0 - 2.0 ** -40), nug)
+    return Kg.reshape(-1).contiguous(), (Kg * nug).reshape(-1).contiguous(), side
+
+
+def _cell_pool(v: torch.Tensor, sub: int) -> torch.Tensor:
+    """Max of v [n, side, side] over each closed cell of the lattice."""
+    return torch.nn.functional.max_pool2d(v[:, None], kernel_size=sub + 1, stride=sub)[:, 0]
+
+
+def lattice_error_map(sur: "Surrogate", bins: int = MAP_BINS, sub: int = LATTICE, chunk: int = 4) -> torch.Tensor:
+    """Worst relative error of the network per process and (element, log10 K
+    bin, log10 nu bin) cell over a closed lattice of (sub + 1)^2 points per
+    cell (the cell's edges and corners included, shared with its
+    neighbours), [4, elements, bins, bins]. A cell where the exact process is
+    open at some lattice points and closed at others contains a kinematic
+    threshold, near which the DCS falls steeply to zero: it reads +inf (always
+    exact); a cell where it is closed at every point reads NO_SAMPLES."""
+    Kf, qf, side = _lattice(bins, sub, sur.net.device)
+    npt = side * side
+    maps = []
+    for e0 in range(0, len(sur.elements), chunk):
+        els = sur.elements[e0:e0 + chunk]
+        off = np.arange(len(els) + 1, dtype=np.int64) * npt
+        K, q = Kf.repeat(len(els)), qf.repeat(len(els))
+        ex = exact_dcs(K, q, off, els)
+        part = Surrogate(sur.net, els, sur.stats, None, sur.tol)
+        out, _ = part.evaluate(K, q, off, recheck_per_mille=0, use_map=False)
+        per = []
+        for p in range(NPROC):
+            e = ex[p].reshape(len(els), side, side)
+            s = out[p].double().reshape(len(els), side, side)
+            rel = torch.where(e > 0, (s - e).abs() / e.clamp_min(1e-300), (s != 0).double())
+            worst = _cell_pool(rel, sub)
+            opened, closed = _cell_pool((e > 0).double(), sub), _cell_pool((e <= 0).double(), sub)
+            m = torch.where(opened == 0, torch.full_like(worst, NO_SAMPLES),
+                            torch.where(closed > 0, torch.full_like(worst, float("inf")), worst))
+            per.append(m.float())
+        maps.append(torch.stack(per))
+        del K, q, ex, out
+    return torch.cat(maps, dim=1)
+
+
+def threshold_cells(elements, dev, bins: int = MAP_BINS, sub: int = LATTICE, chunk: int = 4) -> torch.Tensor:
+    """[4, elements, bins, bins] bool: the cells where the exact process is
+    open at some lattice points and closed at others (a kinematic threshold
+    crosses them). The network never serves them, so training leaves their
+    samples out."""
+    Kf, qf, side = _lattice(bins, sub, dev)
+    npt = side * side
+    maps = []
+    for e0 in range(0, len(elements), chunk):
+        els = elements[e0:e0 + chunk]
+        off = np.arange(len(els) + 1, dtype=np.int64) * npt
+        ex = exact_dcs(Kf.repeat(len(els)), qf.repeat(len(els)), off, els)
+        per = []
+        for p in range(NPROC):
+            e = ex[p].reshape(len(els), side, side)
+            per.append((_cell_pool((e > 0).double(), sub) > 0) & (_cell_pool((e <= 0).double(), sub) > 0))
+        maps.append(torch.stack(per))
+        del ex
+    return torch.cat(maps, dim=1)
+
+
+def cell_index(K: torch.Tensor, q: torch.Tensor, offsets, bins: int = MAP_BINS) -> torch.Tensor:
+    """Flat (element, log10 K bin, log10 nu bin) cell of each pair."""
+    dev = K.device
+    el = torch.empty(K.numel(), dtype=torch.int64, device=dev)
+    for e in range(len(offsets) - 1):
+        el[int(offsets[e]):int(offsets[e + 1])] = e
+    ik = ((torch.log10(K) + 3.0) / 9.0 * bins).long().clamp(0, bins - 1)
+    inu = ((torch.log10(q / K) + 6.0) / 6.0 * bins).long().clamp(0, bins - 1)
+    return (el * bins + ik) * bins + inu
+
+
+def combine_maps(a: torch.Tensor, b: torch.Tensor) -> torch.Tensor:
+    """Cell-wise worst of two error maps; NO_SAMPLES only where both have none."""
+    both_none = (a == NO_SAMPLES) & (b == NO_SAMPLES)
+    return torch.where(both_none, torch.full_like(a, NO_SAMPLES), torch.maximum(a, b))
 
 
 def max_rel_errors(out, exact) -> list[dict]:
@@ -342,24 +429,30 @@ def max_rel_errors(out, exact) -> list[dict]:
     return res
 
 
-def train_surrogate(elements, n_train: int = 1 << 22, n_val: int = 1 << 24, epochs: int = 200, batch: int = 8192,
+def train_surrogate(elements, n_train: int = 1 << 22, n_val: int = 1 << 26, epochs: int = 200, batch: int = 8192,
                     tol: float = 1e-3, seed: int = 0, device=None, log=print, lr_max: float = 2e-3,
-                    lr_min: float = 2e-5) -> tuple[Surrogate, dict]:
+                    lr_min: float = 2e-5, skip_threshold_cells: bool = True) -> tuple[Surrogate, dict]:
     dev = device or torch.device("cuda", 0)
     t0 = time.time()
     off_tr = equal_offsets(len(elements), n_train)
     K, q = make_pairs(off_tr, seed=1000 + seed, device=dev)
     ex = exact_dcs(K, q, off_tr, elements)
     X = features(K, q, off_tr, elements)
-    T, M, stats = targets(ex, elements, off_tr, X)
-    del ex
+    exclude = None
+    if skip_threshold_cells:  # never served: the fit leaves them out
+        tc = threshold_cells(elements, dev).reshape(NPROC, -1)
+        cell = cell_index(K, q, off_tr)
+        exclude = torch.stack([tc[p][cell] for p in range(NPROC)], 1)
+        log(f"threshold cells per process {[round(float(tc[p].float().mean()), 3) for p in range(NPROC)]}")
+    T, M, stats = targets(ex, elements, off_tr, X, exclude=exclude)
+    del ex, exclude
     log(f"dataset {n_train} pairs in {time.time() - t0:.1f}s; active fraction per process "
         f"{[round(float(M[:, p].mean()), 3) for p in range(NPROC)]}")
     net = Network(seed=seed, max_rows=batch, device=dev)
     t1 = time.time()
     # chunks of epochs, each with its piece of one cosine schedule, so that
     # progress is reported while training runs
-    chunk = 10
+    chunk = max(10, epochs // 40)
     cos = lambda f: lr_min + (lr_max - lr_min) * 0.5 * (1 + np.cos(np.pi * f))  # noqa: E731
     losses = []
     for c0 in range(0, epochs, chunk):
@@ -374,15 +467,19 @@ def train_surrogate(elements, n_train: int = 1 << 22, n_val: int = 1 << 24, epoc
     del K, q, X, T, M
     sur = Surrogate(net, elements, stats, None, tol)
     report = {"n_train": n_train, "epochs": epochs, "batch": batch, "train_s": t_train,
+              "skip_threshold_cells": skip_threshold_cells,
               "loss_first": float(losses[0]), "loss_last": float(losses[-1]), "lr_max": lr_max, "lr_min": lr_min}
     report.update(validate(sur, n_val=n_val, seed=seed, log=log))
+    sur.meta = {**sur.meta, **report}
     return sur, report
 
 
-def validate(sur: Surrogate, n_val: int = 1 << 24, seed: int = 0, log=print) -> dict:
-    """Held-out validation (disjoint seed): the per-process, per-cell worst
-    error map the flags derive from (stored in sur.cell_err, flags set for
-    sur.tol), and the raw error figures."""
+def validate(sur: Surrogate, n_val: int = 1 << 26, seed: int = 0, log=print) -> dict:
+    """Held-out validation (disjoint seed) and the lattice check: the
+    per-process, per-cell worst error map the flags derive from (the worst of
+    the held-out samples and of lattice_error_map; stored in sur.cell_err,
+    flags set for sur.tol), and the raw error figures of the held-out
+    samples."""
     dev = sur.net.device
     off_va = equal_offsets(len(sur.elements), n_val)
     Kv, qv = make_pairs(off_va, seed=2000 + seed, device=dev)
@@ -390,6 +487,8 @@ def validate(sur: Surrogate, n_val: int = 1 << 24, seed: int = 0, log=print) -> 
     outv, _ = sur.evaluate(Kv, qv, off_va, recheck_per_mille=0, use_map=False)
     emap = error_map(outv, exv, Kv, qv, off_va)
     raw = max_rel_errors(outv, exv)
+    del Kv, qv, exv, outv
+    emap = combine_maps(emap, lattice_error_map(sur))
     sur.cell_err = emap.cpu().numpy().astype(np.float32)
     sur.set_tolerance(sur.tol)
     worst = emap.max(dim=0).values
@@ -489,7 +588,7 @@ def bench_entry(K, q, ref, steps, warmup, stream, wall_f64, tolerances=(None, 3e
 
 def main() -> None:
     """python -m noa_amd.surrogate [--epochs E] [--out PATH]: trains on the
-    48-element batch (2^22 samples), validates on 2^24 held-out pairs and
+    48-element batch (2^22 samples), validates on 2^26 held-out pairs and
     writes the checkpoint with its error map."""
     import argparse
     import json
@@ -497,10 +596,12 @@ def main() -> None:
     ap.add_argument("--epochs", type=int, default=200)
     ap.add_argument("--batch", type=int, default=8192)
     ap.add_argument("--n-train", type=int, default=1 << 22)
-    ap.add_argument("--n-val", type=int, default=1 << 24)
+    ap.add_argument("--n-val", type=int, default=1 << 26)
     ap.add_argument("--tol", type=float, default=1e-3)
     ap.add_argument("--lr", type=float, default=2e-3, help="peak learning rate of the cosine schedule")
     ap.add_argument("--out", default=CHECKPOINT)
+    ap.add_argument("--keep-threshold-cells", action="store_true",
+                    help="train on the samples of cells crossed by a kinematic threshold too")
     ap.add_argument("--revalidate", metavar="CHECKPOINT",
                     help="recompute the validation error map of a trained checkpoint and write it to --out")
     a = ap.parse_args()
@@ -513,7 +614,8 @@ def main() -> None:
         return
     elements, _ = workload.element_batch(48, seed=0)
     sur, report = train_surrogate(elements, n_train=a.n_train, n_val=a.n_val, epochs=a.epochs, batch=a.batch,
-                                  tol=a.tol, log=lambda s: print(s, flush=True), lr_max=a.lr, lr_min=a.lr / 100)
+                                  tol=a.tol, log=lambda s: print(s, flush=True), lr_max=a.lr, lr_min=a.lr / 100,
+                                  skip_threshold_cells=not a.keep_threshold_cells)
     sur.save(a.out)
     print(json.dumps(report), flush=True)
 

@@ -181,8 +181,8 @@ def test_trained_network_gain(device):
     zero) and every cell is routed to the exact kernel. The trained network
     (1) cuts the median error of the trend ~100x and its p99 at least 2x per
     process, and (2) at 3e-2 serves most validation cells itself (none at the
-    stated 1e-3, where every query is exact); served queries stay within the
-    tolerance up to rare outliers that the seeded recheck reports."""
+    stated 1e-3, where every query is exact), every query within the
+    tolerance."""
     sur = S.Surrogate.load(device=device)
     assert sur.cell_err is not None
     off = S.equal_offsets(len(sur.elements), 1 << 22)
@@ -226,6 +226,5 @@ def test_trained_network_gain(device):
     for p in range(4):
         nz = ex[p] > 0
         e = (out[p].double()[nz] / ex[p][nz] - 1).abs()
-        frac_over = (e > 3e-2).double().mean().item()
-        print(f"process {p}: max {e.max().item():.3e}, share above 3e-2 {frac_over:.2e}", flush=True)
-        assert frac_over < 1e-3, p
+        print(f"process {p}: max {e.max().item():.3e}", flush=True)
+        assert e.max().item() <= 3e-2, p
