@@ -353,6 +353,8 @@ def main() -> None:
             forms["surrogate"] = surrogate.bench_entry(K, q, ref, args.steps, args.warmup, stream, wall)
         except (ImportError, FileNotFoundError) as e:
             forms["surrogate"] = {"skipped": str(e)}
+        from noa_amd import surrogate
+        forms["surrogate_training"] = surrogate.training_entry(dev)
         del K, q, outs, ref
         torch.cuda.empty_cache()
         from noa_amd.batch import BatchPlan

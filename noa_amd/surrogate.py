@@ -535,6 +535,45 @@ def smoke(dev) -> str:
     return f"surrogate smoke ok (loss {losses[0]:.3f} -> {losses[-1]:.3f}, fused-vs-layered {d:.1e})"
 
 
+# Dense multiply-add flops of one forward pass: 2 (3 x 128 + 3 x 128 x 128 + 128 x 4).
+FORWARD_FLOPS = 2 * (3 * 128 + 3 * 128 * 128 + 128 * 4)
+# One training sample: the forward pass, the activation gradients and the weight gradients.
+TRAIN_FLOPS = 3 * FORWARD_FLOPS
+BF16_MFMA_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 matrix peak
+
+
+def training_entry(dev, epochs: int = 2, batch: int = 8192, n: int = 1 << 22) -> dict:
+    """Training throughput at the stated workload: a fresh network on 2^22
+    samples of the 48-element batch labelled by the FP64 kernel, `epochs`
+    epochs at `batch` rows per step (three fused kernels per step), timed
+    with HIP events on the stream; samples/s and the MFMA roofline."""
+    elements, _ = workload.element_batch(48, seed=0)
+    off = equal_offsets(len(elements), n)
+    K, q = make_pairs(off, seed=1000, device=dev)
+    ex = exact_dcs(K, q, off, elements)
+    X = features(K, q, off, elements)
+    T, M, _ = targets(ex, elements, off, X)
+    del ex, K, q
+    net = Network(seed=0, max_rows=batch, device=dev)
+    net.train(X, T, M, epochs=1, batch=batch)  # warm-up
+    stream = torch.cuda.current_stream(dev)
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    s.record(stream)
+    losses = net.train(X, T, M, epochs=epochs, batch=batch, seed=1)
+    e.record(stream)
+    torch.cuda.synchronize()
+    sec = s.elapsed_time(e) * 1e-3
+    net.close()
+    steps = epochs * (n // batch)
+    rate = steps * batch / sec
+    achieved = rate * TRAIN_FLOPS / 1e12
+    return {"samples_per_s": rate, "ms_per_step": sec * 1e3 / steps, "batch": batch, "n_train": n,
+            "epochs": epochs, "loss_per_epoch": [float(x) for x in losses],
+            "roofline": {"bound": "mfma", "achieved": achieved, "peak": BF16_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": achieved / BF16_MFMA_PEAK_TFLOPS, "flops_per_sample": TRAIN_FLOPS}}
+
+
 def bench_entry(K, q, ref, steps, warmup, stream, wall_f64, tolerances=(None, 3e-2)) -> dict:
     """Config-4 style measurement: the committed checkpoint on 2^28 queries
     over its 48 elements, 1% seeded recheck, flagged cells exact; errors
@@ -562,6 +601,21 @@ def bench_entry(K, q, ref, steps, warmup, stream, wall_f64, tolerances=(None, 3e
     dt_exact = (time.perf_counter() - t) / reps
     stated = sur.tol
     res = {"n_queries": n, "exact_f64_evals_per_s_same_queries": n / dt_exact, "stated_tolerance": stated}
+    # every query through the network (no flags, no recheck): the inference
+    # kernel's roofline against the dense BF16 MFMA peak
+    for _ in range(max(1, warmup // 2)):
+        sur.evaluate(Kq, qq, off, recheck_per_mille=0, use_map=False, outputs=outs, workspace=ws)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        sur.evaluate(Kq, qq, off, recheck_per_mille=0, use_map=False, outputs=outs, workspace=ws)
+    torch.cuda.synchronize()
+    dt_net = (time.perf_counter() - t) / reps
+    achieved = n * FORWARD_FLOPS / dt_net / 1e12
+    res["network_only"] = {"evals_per_s": n / dt_net, "speedup_vs_f64": dt_exact / dt_net,
+                           "roofline": {"bound": "mfma", "achieved": achieved, "peak": BF16_MFMA_PEAK_TFLOPS,
+                                        "unit": "TFLOP/s", "frac": achieved / BF16_MFMA_PEAK_TFLOPS,
+                                        "flops_per_query": FORWARD_FLOPS}}
     for tol in tolerances:
         tol = stated if tol is None else tol
         if sur.cell_err is not None:
