@@ -231,6 +231,47 @@ def consumers(dev, stream, steps: int) -> dict:
     return out
 
 
+def notebook_shape(dev) -> dict:
+    """The reference's one published timing (docs/pms/muon_dcs_calc.ipynb
+    cells 12-13, 21-22): one bremsstrahlung map over K = linspace(1e-3, 1e6,
+    10000), q = 0.0505 K, standard rock, through the notebook's Python
+    module (here noa_amd._muon_dcs), the output allocated per call. %timeit
+    there: 155 us per call on CUDA, 295 us per call on the CPU (serial). Here:
+    per call over 10000 back-to-back calls with one synchronisation at the end
+    (what %timeit measures on an asynchronous device), per call with a
+    synchronisation after each, and the CPU tensors through the same module."""
+    import torch
+    from noa_amd import _muon_dcs
+    K = torch.linspace(1e-3, 1e6, 10000, dtype=torch.float64)
+    q = 0.0505 * K
+    Kd, qd = K.to(dev), q.to(dev)
+    f = lambda a, b: _muon_dcs.bremsstrahlung(a, b)  # noqa: E731
+    for _ in range(100):
+        f(Kd, qd)
+    torch.cuda.synchronize()
+    n = 10000
+    t0 = time.perf_counter()
+    for _ in range(n):
+        f(Kd, qd)
+    torch.cuda.synchronize()
+    async_us = (time.perf_counter() - t0) / n * 1e6
+    t0 = time.perf_counter()
+    for _ in range(1000):
+        f(Kd, qd)
+        torch.cuda.synchronize()
+    sync_us = (time.perf_counter() - t0) / 1000 * 1e6
+    for _ in range(10):
+        f(K, q)
+    t0 = time.perf_counter()
+    for _ in range(1000):
+        f(K, q)
+    cpu_us = (time.perf_counter() - t0) / 1000 * 1e6
+    return {"gpu_us_per_call": async_us, "gpu_us_per_call_synchronised": sync_us, "cpu_us_per_call": cpu_us,
+            "published_cuda_us_per_call": 155.0, "published_cpu_us_per_call": 295.0,
+            "config": "bremsstrahlung only, N = 10000 FP64 pairs, K = linspace(1e-3, 1e6), q = 0.0505 K, output "
+                      "allocated per call, through the notebook's module (docs/pms/muon_dcs_calc.ipynb cell 21-22)"}
+
+
 def pumas_table_entry(dev, stream, steps: int, with_cpu: bool) -> dict:
     """PUMAS v1.2.1 DCS tabulation (compute_dcs_table) for the 48-element
     batch's elements at 146 kinetic energies 1e-3..1e6 GeV, muon, cutoff 0.05:
@@ -348,6 +389,10 @@ def main() -> None:
         except ImportError as e:
             forms["libtorch_map_all_f64"] = {"skipped": str(e)}
         forms.update(consumers(dev, stream, max(3, args.steps // 10)))
+        try:
+            forms["notebook_shape_bremsstrahlung_1e4"] = notebook_shape(dev)
+        except ImportError as e:
+            forms["notebook_shape_bremsstrahlung_1e4"] = {"skipped": str(e)}
         try:
             from noa_amd import surrogate
             forms["surrogate"] = surrogate.bench_entry(K, q, ref, args.steps, args.warmup, stream, wall)
