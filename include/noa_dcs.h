@@ -110,12 +110,17 @@ int noa_dcs_eval_f64_host(const double* K, const double* q, int64_t n, const noa
  *            elsewhere (table_get_dcs);
  *   x        [n_x] floats, the nodes ln(q / K) (table_DCS_x), or NULL;
  *   envelope [3][n_elements][n_K][2] floats, or NULL: entry 0 the envelope
- *            exponent (with the low-energy fill of pumas.c:11745-11761);
- *            entry 1, the envelope maximum, is written as 0 (not computed);
+ *            exponent (with the low-energy fill of pumas.c:11745-11761),
+ *            entry 1 the envelope maximum, max over the row's kinematic
+ *            range above the cutoff of DCS x^exponent, the DCS interpolated
+ *            from the table as PUMAS evaluates it (dcs_tabulate_envelope_row,
+ *            pumas.c:11526-11679);
  *   table_f64 the table in FP64 before its float rounding, or NULL.
- * Rows are not gated by the cross-section normalisation (pumas.c:11427-11434
- * zeroes a row whose total cross section is 0): callers that hold it apply
- * it. Work is enqueued on `stream`.
+ * csn, the cross-section normalisation table_CSn [3][n_elements][n_K]
+ * (device doubles), gates rows as PUMAS does: a row whose value is not
+ * positive is all zeros and has no envelope (pumas.c:11427-11434, 11532-11536;
+ * its exponent is written as 0). NULL: every row positive. Work is enqueued
+ * on `stream`.
  *
  * noa_pumas_dcs_f64: the model DCS of one process (m^2/kg, the wrappers of
  * pumas.c:11067-11112) over n device (K, q) pairs.
@@ -123,7 +128,7 @@ int noa_dcs_eval_f64_host(const double* K, const double* q, int64_t n, const noa
 int noa_pumas_n_table_dcs(double cutoff);
 int noa_pumas_dcs_table(const double* Z, const double* A, int32_t n_elements, double mass, double cutoff,
                         const double* K, int32_t n_K, float* table, float* x, float* envelope, double* table_f64,
-                        hipStream_t stream);
+                        const double* csn, hipStream_t stream);
 int noa_pumas_dcs_f64(int32_t process, double Z, double A, double mass, const double* K, const double* q, int64_t n,
                       double* out, hipStream_t stream);
 

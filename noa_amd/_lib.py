@@ -94,7 +94,7 @@ SIGNATURES = {
                                                             ctypes.POINTER(OutputsF64), ctypes.POINTER(OutputsF64),
                                                             _P]),
     "noa_pumas_n_table_dcs": (ctypes.c_int, [_D]),
-    "noa_pumas_dcs_table": (ctypes.c_int, [_P, _P, _I32, _D, _D, _P, _I32, _P, _P, _P, _P, _P]),
+    "noa_pumas_dcs_table": (ctypes.c_int, [_P, _P, _I32, _D, _D, _P, _I32, _P, _P, _P, _P, _P, _P]),
     "noa_pumas_dcs_f64": (ctypes.c_int, [_I32, _D, _D, _D, _P, _P, _I64, _P, _P]),
     "noa_dcs_mlp_num_params": (ctypes.c_int64, []),
     "noa_dcs_mlp_create": (ctypes.c_int, [_U64, _I32, ctypes.POINTER(_P)]),

@@ -14,9 +14,8 @@
 // ends and the linear first bin of math_pchip_initialise, pumas.c:10039-10088)
 // and the envelope exponent. A second launch gives the rows below the first
 // positive exponent of each process and element that exponent
-// (pumas.c:11745-11761). The envelope maximum (pumas.c:11526-11679, a
-// minimisation over the interpolated table) is not computed: entry 1 of each
-// envelope pair is written as 0.
+// (pumas.c:11745-11761), and a third the envelope maxima (pumas.c:11526-11679,
+// a search over the interpolated table; see pumas_envelope_kernel).
 //
 // FP64 throughout, with the device libm (log, exp, pow, sqrt, log10); the
 // grid is computed on the host with the reference's operations.
@@ -444,7 +443,8 @@ __global__ __launch_bounds__(64) void pumas_table_kernel(Grid g, ElementBlock bl
                                                          int32_t n_blk, int32_t n_el, double mass,
                                                          const double* __restrict__ Ks, int32_t n_K,
                                                          float* __restrict__ table, double* __restrict__ table64,
-                                                         float* __restrict__ x_out, float* __restrict__ env) {
+                                                         float* __restrict__ x_out, float* __restrict__ env,
+                                                         const double* __restrict__ csn) {
 #pragma clang fp contract(off)
     __shared__ double ys[MAX_X];
     __shared__ double ms[MAX_X];
@@ -458,6 +458,22 @@ __global__ __launch_bounds__(64) void pumas_table_kernel(Grid g, ElementBlock bl
     const ElementConsts& c = blk.e[j];
     const double K = Ks[k];
     const int lane = threadIdx.x;
+    if (csn && !(csn[r] > 0.)) {  // no cross section: an empty row (pumas.c:11427-11434)
+        for (int i = lane; i < n; i += 64) {
+            table[r * 2 * n + 2 * i] = 0.f;
+            table[r * 2 * n + 2 * i + 1] = 0.f;
+            if (table64) {
+                table64[r * 2 * n + 2 * i] = 0.;
+                table64[r * 2 * n + 2 * i + 1] = 0.;
+            }
+            if (b == 0 && x_out) x_out[i] = float(g.x[i]);
+        }
+        if (lane == 0 && env) {  // the reference leaves the pair as it was; here 0
+            env[2 * r] = 0.f;
+            env[2 * r + 1] = 0.f;
+        }
+        return;
+    }
 
     for (int i = lane; i < n; i += 64) {
         const double q = K * exp(g.x[i]);
@@ -589,6 +605,419 @@ __global__ __launch_bounds__(64) void pumas_alpha_fill_kernel(float* __restrict_
     for (int row = 0; row < n0; row++) a[2 * row] = alpha;
 }
 
+// The m^2/kg model DCS for a whole wave that evaluates one (K, q): every lane
+// computes the same set-up, the quadrature nodes are spread over the lanes
+// (12 for pair production, 9 for photonuclear), and every lane then sums the
+// gathered node terms in the serial loop's order, so the value is that of
+// dcs() bit for bit. Must be called with the wave's 64 lanes active and the
+// same arguments in every lane.
+__device__ double dcs_wave(int process, const ElementConsts& c, const Globals& G, double mass, double K, double q) {
+#pragma clang fp contract(off)
+    const int lane = threadIdx.x & 63;
+    double d;
+    if (process == 0) {
+        d = brems_ssr(c, G, mass, K, q);
+    } else if (process == 1) {
+        if ((c.Z <= 0) || (c.A <= 0) || (mass <= 0) || (K <= 0) || (q <= 0)) return 0.;
+        if (q <= 4. * ELECTRON_MASS) return 0.;
+        if (q >= K + c.p_qoff) return 0.;
+        const double gamma = 1. + K / mass;
+        const double x0 = 1. - 4. * ELECTRON_MASS / q;
+        const double x1 = 1. - 6. / (gamma * (gamma - q / mass));
+        if (!((x0 > 0) && (x1 > 0))) return 0.;
+        const double rmax = sqrt(x0) * x1;
+        const double ri = 1. - rmax;
+        if ((ri <= 0.) || (ri >= 1.)) return 0.;
+        const double tmin = log(ri);
+        const PairQ s = pair_setup(c, mass, K, q);
+        double term = 0.;
+        if (lane < 12) {
+            const double rho = exp(GQ12_X[lane] * tmin);
+            term = pair_d2_ssr(s, G, rho) * rho * GQ12_W[lane] * tmin;
+        }
+        double I = 0.;
+        for (int i = 0; i < 12; i++) I -= __shfl(term, i, 64);
+        d = (I < 0) ? 0. : I;
+    } else {
+        const double M = NUCLEON_MASS;
+        const double mpi = PION_MASS;
+        const double ml = mass;
+        if ((q < mpi + 0.5 * mpi * mpi / M) || (q > K + ml - 0.5 * (M + ml * ml / M))) return 0.;
+        if ((c.Z <= 0) || (c.A <= 0) || (ml <= 0) || (K <= 0)) return 0.;
+        if ((q <= mpi + 0.5 * mpi * mpi / M) || (q >= K + ml - 0.5 * (M + ml * ml / M))) return 0.;
+        const double E = K + ml;
+        const double ml2 = ml * ml;
+        const double Q2min = ml2 * (q * q - 0.5 * ml2) / (E * (E - q));
+        const double Q2max = 2.0 * M * (q - mpi) - mpi * mpi;
+        if ((Q2max < Q2min) | (Q2min < 0)) return 0.;
+        const double lmin = log(Q2min);
+        const double dx = log(Q2max) - lmin;
+        double term = 0.;
+        int skip = 1;
+        if (lane < 9) {
+            const double Q2 = exp(lmin + dx * GQ9_X[lane]);
+            const double tmp = photo_d2_drss(c, G, ml, K, q, Q2);
+            skip = tmp <= 0.;
+            term = tmp * (GQ9_W[lane] * (dx * Q2));
+        }
+        double ds = 0.;
+        for (int i = 0; i < 9; i++) {
+            const double t = __shfl(term, i, 64);
+            if (!__shfl(skip, i, 64)) ds += t;
+        }
+        d = (ds < 0.) ? 0. : ds;
+    }
+    return d * 1E+03 * AVOGADRO_NUMBER * (mass + K) / c.A;
+}
+
+// ---- the envelope maximum (dcs_tabulate_envelope_row, pumas.c:11526-11679) ----
+//
+// One wave per row: the reference's serial search (bisections that find where the DCS
+// becomes positive at the range ends, then two bracketing bisections,
+// math_find_minimum with algo 0, pumas.c:12172-12232 and 12416-12459, and for
+// pair production at K >= 1e7 a logarithmic scan next to the upper end) of
+// the maximum of DCS x^alpha over ln x, where DCS is the table interpolated as
+// dcs_evaluate does without a context (pumas.c:11307-11398) and the model
+// where the table does not apply. The rows read are those of the finished
+// table. The search's control flow is the same in every lane of the wave
+// (same row, same values), and the model evaluations spread their quadrature
+// nodes over the lanes (dcs_wave), so rows do not diverge against each other. The reference's search reads the nodes from physics->table_DCS_x,
+// which compute_dcs_table writes only after element 0 (pumas.c:11775-11783),
+// so element 0's search sees zero nodes and takes the model everywhere; the
+// kernel does the same.
+
+struct EnvRow {
+    const ElementConsts* c;
+    const Globals* G;
+    int process;
+    double mass;
+    const float* xt;    // the nodes as the search sees them (LDS)
+    int n;
+    // dcs_evaluate's energy bracket: K is the row's own energy at every
+    // evaluation, so the two rows it reads (LDS copies) and their weights
+    // are fixed per row
+    const float* ra;  // row i0 (or the last row)
+    const float* rb;  // row i0 + 1 (or the last row again)
+    float h0, h1;
+    bool below;  // K below the grid (never the case for a row's own energy)
+};
+
+__device__ __forceinline__ void bracketf(const float* table, float value, int* p1, int* p2) {
+    while (true) {
+        const int i3 = (*p1 + *p2) / 2;
+        if (value >= table[i3])
+            *p1 = i3;
+        else
+            *p2 = i3;
+        if (*p2 - *p1 < 2) break;
+    }
+}
+
+// table_index without a context (pumas.c:4249-4290, 4299-4306).
+__device__ __forceinline__ int table_index(const double* table, int n_K, double value) {
+    const int imax = n_K - 1;
+    if (value < table[0]) return -1;
+    if (value >= table[imax]) return imax;
+    int i1 = 0, i2 = imax;
+    while (true) {
+        const int i3 = (i1 + i2) / 2;
+        if (value >= table[i3])
+            i1 = i3;
+        else
+            i2 = i3;
+        if (i2 - i1 < 2) break;
+    }
+    return i1;
+}
+
+__device__ double dcs_interpolated(const EnvRow& a, double K, double q) {
+#pragma clang fp contract(off)
+    const double wj = K / (K + a.mass);
+    const ElementConsts& c = *a.c;
+    double qlow = 0., qmin, qmax;
+    if (a.process == 2) {
+        qmin = PION_MASS + 0.5 * PION_MASS * PION_MASS / NUCLEON_MASS;
+        qmax = K + a.mass - 0.5 * (NUCLEON_MASS + a.mass * a.mass / NUCLEON_MASS);
+        if ((q < qmin) || (q > qmax)) return 0.;
+        qlow = 2 * qmin;
+    } else {
+        qmin = 4. * ELECTRON_MASS;
+        qmax = K + c.p_qoff;
+        if (a.process == 1)
+            qlow = 4 * qmin;
+        else
+            qmin = 0.;
+        if ((q < qmin) || (q > qmax)) return 0.;
+    }
+    if (q < qlow) return dcs_wave(a.process, c, *a.G, a.mass, K, q) * wj;
+    const int nx = a.n;
+    const float xmin = a.xt[0], xmax = a.xt[nx - 1];
+    const float x = logf((float)(q / K));
+    if ((x < xmin) || (x >= xmax)) return dcs_wave(a.process, c, *a.G, a.mass, K, q) * wj;
+    int ix = 0, tmpi = nx - 1;
+    bracketf(a.xt, x, &ix, &tmpi);
+    if ((ix < 0) || (ix >= nx - 1)) return dcs_wave(a.process, c, *a.G, a.mass, K, q) * wj;
+    if (a.below) return dcs_wave(a.process, c, *a.G, a.mass, K, q) * wj;
+    float p00 = a.ra[2 * ix], m00 = a.ra[2 * ix + 1], p01 = a.ra[2 * ix + 2], m01 = a.ra[2 * ix + 3];
+    float p10 = a.rb[2 * ix], m10 = a.rb[2 * ix + 1], p11 = a.rb[2 * ix + 2], m11 = a.rb[2 * ix + 3];
+    const float h0 = a.h0, h1 = a.h1;
+    if (((p00 == 0.f) && (m00 == 0.f)) || ((p01 == 0.f) && (m01 == 0.f)) || ((p10 == 0.f) && (m10 == 0.f)) ||
+        ((p11 == 0.f) && (m11 == 0.f)))
+        return dcs_wave(a.process, c, *a.G, a.mass, K, q) * wj;
+    const float x0 = a.xt[ix];
+    const float x1 = a.xt[ix + 1];
+    const float dx = x1 - x0;
+    m00 *= dx;
+    m01 *= dx;
+    const float t = (x - x0) / dx;
+    const float c02 = -3 * (p00 - p01) - 2 * m00 - m01;
+    const float c03 = 2 * (p00 - p01) + m00 + m01;
+    float r = expf(p00 + t * (m00 + t * (c02 + t * c03))) * h0;
+    if (h1 > 0.f) {
+        m10 *= dx;
+        m11 *= dx;
+        const float c12 = -3 * (p10 - p11) - 2 * m10 - m11;
+        const float c13 = 2 * (p10 - p11) + m10 + m11;
+        r += expf(p10 + t * (m10 + t * (c12 + t * c13))) * h1;
+    }
+    return r * wj;
+}
+
+struct EnvObjective {
+    const EnvRow& a;
+    double K, alpha;
+    __device__ double operator()(double lnx) const {
+#pragma clang fp contract(off)
+        const double x = exp(lnx);
+        const double q = x * K;
+        return -dcs_interpolated(a, K, q) * pow(x, alpha);
+    }
+};
+
+template <class F>
+__device__ int find_minimum_bisection(const F& f, double xa, double xb, double xc, const double* fb_p, double tol,
+                                      int max_iter, double* fopt) {
+#pragma clang fp contract(off)
+    double fb = fb_p ? *fb_p : f(xb);
+    tol *= fabs(xc - xa);
+    int i = 0;
+    while ((xc - xa > tol) && (i < max_iter)) {
+        if (xb - xa > xc - xb) {
+            const double x = 0.5 * (xa + xb);
+            const double fi = f(x);
+            if (fi < fb) {
+                xc = xb;
+                xb = x;
+                fb = fi;
+            } else {
+                xa = x;
+            }
+        } else {
+            const double x = 0.5 * (xb + xc);
+            const double fi = f(x);
+            if (fi < fb) {
+                xa = xb;
+                xb = x;
+                fb = fi;
+            } else {
+                xc = x;
+            }
+        }
+        i++;
+    }
+    *fopt = fb;
+    return (i <= max_iter) ? i : -1;
+}
+
+template <class F>
+__device__ int find_minimum(const F& f, double xa, double xc, const double* fa_p, const double* fc_p, double tol,
+                            int max_iter, double* fopt) {
+#pragma clang fp contract(off)
+    double fa = fa_p ? *fa_p : f(xa);
+    double fc = fc_p ? *fc_p : f(xc);
+    const double atol = tol * fabs(xc - xa);
+    double xb = 0, fb = 0;
+    int i;
+    for (i = 0; i < max_iter; i++) {
+        xb = 0.5 * (xa + xc);
+        fb = f(xb);
+        if ((fb < fa) && (fb < fc)) break;
+        if (fabs(xc - xa) <= atol) {
+            *fopt = fb;
+            return i;
+        }
+        if (fb < fc) {
+            xc = xb;
+            fc = fb;
+        } else {
+            xa = xb;
+            fa = fb;
+        }
+    }
+    if (i == max_iter) {
+        if (fa < fb) {
+            fb = fa;
+        } else if (fc < fb) {
+            fb = fc;
+        }
+        *fopt = fb;
+        return -1;
+    }
+    const int status = find_minimum_bisection(f, xa, xb, xc, &fb, tol, max_iter - i, fopt);
+    return (status < 0) ? -2 : status + i;
+}
+
+__device__ float envelope_max(const EnvRow& a, double K, double cutoff, float alpha, double cs) {
+#pragma clang fp contract(off)
+    constexpr double RESOLUTION = 1E-10;
+    if (cs <= 0.) return 0.f;
+    double qmin, qmax;
+    if (a.process == 2) {  // dcs_get_range (pumas.c:11047-11056)
+        qmin = PION_MASS + 0.5 * PION_MASS * PION_MASS / NUCLEON_MASS;
+        qmax = K + a.mass - 0.5 * (NUCLEON_MASS + a.mass * a.mass / NUCLEON_MASS);
+    } else {
+        qmin = (a.process == 0) ? 0. : 4. * ELECTRON_MASS;
+        qmax = K + a.c->p_qoff;
+    }
+    double xmin = qmin / K;
+    if (xmin < cutoff) xmin = cutoff;
+    xmin = log(xmin);
+    double xmax = log(qmax / K);
+    if (xmin >= xmax) return 0.f;
+    const EnvObjective f{a, K, (double)alpha};
+    double f0 = f(xmin);
+    double f1 = f(xmax);
+    if (f0 == 0.) {
+        double x0 = xmin, x1 = xmax;
+        while (x1 - x0 > RESOLUTION) {
+            const double x2 = 0.5 * (x0 + x1);
+            const double f2 = f(x2);
+            if (f2 != 0) {
+                x1 = xmin = x2;
+                f0 = f2;
+            } else {
+                x0 = x2;
+            }
+        }
+    }
+    if (f1 == 0.) {
+        double x0, x1;
+        if (K >= 1E+01) {
+            const double xm = exp(xmax);
+            double delta = RESOLUTION, x = xmax, fx = f1;
+            while (xm > delta) {
+                x = log(xm - delta);
+                fx = f(x);
+                if (fx != 0.)
+                    break;
+                else
+                    delta += delta;
+            }
+            f1 = fx;
+            x0 = xmax = x;
+            x1 = log(xm - delta / 2);
+        } else {
+            x0 = xmin;
+            x1 = xmax;
+        }
+        while (x1 - x0 > RESOLUTION) {
+            const double x2 = 0.5 * (x0 + x1);
+            const double f2 = f(x2);
+            if (f2 != 0) {
+                x0 = xmax = x2;
+                f1 = f2;
+            } else {
+                x1 = x2;
+            }
+        }
+    }
+    const double xm = 0.5 * (xmin + xmax);
+    const double fm = f(xm);
+    double fopt0 = 0., fopt1 = 0.;
+    find_minimum(f, xmin, xm, &f0, &fm, RESOLUTION, 100, &fopt0);
+    find_minimum(f, xm, xmax, &fm, &f1, RESOLUTION, 100, &fopt1);
+    if ((a.process == 1) && (K >= 1E+07)) {  // a sharp peak next to xmax: a logarithmic scan
+        constexpr int N_SCAN = 7;
+        double xs[N_SCAN], fs[N_SCAN], fmin = 0.;
+        int imin = -1;
+        for (int i = 0; i < N_SCAN; i++) {
+            xs[i] = xmax + log(1. - pow(10., double(-(i + 1))));
+            fs[i] = f(xs[i]);
+            if (fs[i] < fmin) {
+                imin = i;
+                fmin = fs[i];
+            }
+        }
+        if ((imin > 0) && (imin < N_SCAN - 1)) {
+            double fopt2 = 0.;
+            find_minimum_bisection(f, xs[imin - 1], xs[imin], xs[imin + 1], fs + imin, RESOLUTION, 100, &fopt2);
+            if (fopt2 < fopt1) fopt1 = fopt2;
+        }
+    }
+    double fopt = (fopt0 < fopt1) ? fopt0 : fopt1;
+    if (f0 < fopt) fopt = f0;
+    if (f1 < fopt) fopt = f1;
+    return (float)(-fopt);
+}
+
+// One 64-lane workgroup per (process, block element j, kinetic energy k) row
+// of the launch's element block; env[2 r + 1] of table row r.
+__global__ __launch_bounds__(64) void pumas_envelope_kernel(Grid g, ElementBlock blk, Globals G, int32_t e0,
+                                                            int32_t n_blk, int32_t n_el, double mass,
+                                                            const double* __restrict__ Ks, int32_t n_K,
+                                                            const float* __restrict__ table,
+                                                            const double* __restrict__ csn, float* __restrict__ env) {
+    __shared__ float xt[MAX_X];
+    __shared__ float ra[2 * MAX_X], rb[2 * MAX_X];
+    __shared__ ElementConsts sc;  // the row's element and the globals, in LDS for the search's calls
+    __shared__ Globals sg;
+    const int64_t b = blockIdx.x;
+    const int p = int(b / (int64_t(n_blk) * n_K));
+    const int j = int((b / n_K) % n_blk);
+    const int k = int(b % n_K);
+    const int e = e0 + j;
+    const int64_t pe = int64_t(p) * n_el + e;
+    const int64_t r = pe * n_K + k;
+    for (int i = threadIdx.x; i < g.n; i += 64) xt[i] = (e == 0) ? 0.f : (float)g.x[i];
+    if (threadIdx.x == 0) {
+        sc = blk.e[j];
+        sg = G;
+    }
+    // dcs_evaluate's energy bracket of the row's own energy (pumas.c:11352-11381)
+    const double K = Ks[k];
+    const float* rows = table + pe * n_K * 2 * g.n;
+    const int imax = n_K - 1;
+    int i0 = 0, ia = imax, ib = imax;
+    float h0 = 1.f, h1 = 0.f;
+    if (!((K >= Ks[imax]) || ((i0 = table_index(Ks, n_K, K)) >= imax)) && i0 >= 0) {
+        ia = i0;
+        ib = i0 + 1;
+        const float K0 = (float)Ks[i0];
+        const float K1 = (float)Ks[i0 + 1];
+        h1 = logf(((float)K) / K0) / logf(K1 / K0);
+        h0 = 1.f - h1;
+    }
+    for (int i = threadIdx.x; i < 2 * g.n; i += 64) {
+        ra[i] = rows[int64_t(ia) * 2 * g.n + i];
+        rb[i] = rows[int64_t(ib) * 2 * g.n + i];
+    }
+    __syncthreads();
+    EnvRow a;
+    a.c = &sc;
+    a.G = &sg;
+    a.process = p;
+    a.mass = mass;
+    a.n = g.n;
+    a.xt = xt;
+    a.ra = ra;
+    a.rb = rb;
+    a.h0 = h0;
+    a.h1 = h1;
+    a.below = i0 < 0;
+    const float v = envelope_max(a, K, g.cutoff, env[2 * r], csn ? csn[r] : 1.);
+    if (threadIdx.x == 0) env[2 * r + 1] = v;
+}
+
 // The grid (pumas.c:1548-1554, 11712-11737), on the host with its libm.
 inline int n_table(double cutoff) {
     int n = N_REVERSE;
@@ -705,7 +1134,7 @@ int noa_pumas_dcs_f64(int32_t process, double Z, double A, double mass, const do
 
 int noa_pumas_dcs_table(const double* Z, const double* A, int32_t n_elements, double mass, double cutoff,
                         const double* K, int32_t n_K, float* table, float* x, float* envelope, double* table_f64,
-                        hipStream_t stream) {
+                        const double* csn, hipStream_t stream) {
     const int n = noa_pumas_n_table_dcs(cutoff);
     if (n < 0) return n;
     if (n_elements < 0 || n_K < 0) return fail(NOA_DCS_EINVAL, "n_elements and n_K must be >= 0");
@@ -725,7 +1154,7 @@ int noa_pumas_dcs_table(const double* Z, const double* A, int32_t n_elements, do
         for (int j = 0; j < n_blk; j++) blk.e[j] = pumas::make_element(Z[e0 + j], A[e0 + j], mass, G);
         const unsigned rows = unsigned(3 * n_blk * n_K);
         hipLaunchKernelGGL(pumas::pumas_table_kernel, dim3(rows), dim3(64), 0, stream, g, blk, G, e0, n_blk,
-                           n_elements, mass, K, n_K, table, table_f64, e0 == 0 ? x : nullptr, envelope);
+                           n_elements, mass, K, n_K, table, table_f64, e0 == 0 ? x : nullptr, envelope, csn);
         if (int rc = check_hip(hipGetLastError(), "pumas_table_kernel")) return rc;
     }
     if (envelope) {
@@ -733,6 +1162,16 @@ int noa_pumas_dcs_table(const double* Z, const double* A, int32_t n_elements, do
         hipLaunchKernelGGL(pumas::pumas_alpha_fill_kernel, dim3(unsigned((n_pe + 63) / 64)), dim3(64), 0, stream,
                            envelope, n_pe, n_K);
         if (int rc = check_hip(hipGetLastError(), "pumas_alpha_fill_kernel")) return rc;
+        // the envelope maxima, over the finished rows and exponents
+        for (int e0 = 0; e0 < n_elements; e0 += pumas::MAX_EL) {
+            const int n_blk = n_elements - e0 < pumas::MAX_EL ? n_elements - e0 : pumas::MAX_EL;
+            pumas::ElementBlock blk{};
+            for (int j = 0; j < n_blk; j++) blk.e[j] = pumas::make_element(Z[e0 + j], A[e0 + j], mass, G);
+            const unsigned rows = unsigned(3 * n_blk * n_K);
+            hipLaunchKernelGGL(pumas::pumas_envelope_kernel, dim3(rows), dim3(64), 0, stream, g, blk, G, e0, n_blk,
+                               n_elements, mass, K, n_K, table, csn, envelope);
+            if (int rc = check_hip(hipGetLastError(), "pumas_envelope_kernel")) return rc;
+        }
     }
     return ok();
 }

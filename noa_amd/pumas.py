@@ -60,10 +60,17 @@ def dcs(process: int, Z: float, A: float, mass: float, K: torch.Tensor, q: torch
     return out
 
 
-def dcs_table(Z, A, mass: float, cutoff: float, K: torch.Tensor, fp64: bool = False):
+def dcs_table(Z, A, mass: float, cutoff: float, K: torch.Tensor, fp64: bool = False, csn=None):
     """compute_dcs_table for every element (Z[e], A[e]) at the kinetic
-    energies K: (table, x, envelope), plus the FP64 table when fp64."""
+    energies K: (table [3, n_el, n_K, n_x, 2], x [n_x], envelope [3, n_el, n_K,
+    2]: exponent and maximum), plus the FP64 table when fp64. csn (a device
+    tensor [3, n_el, n_K] of doubles, or None: every row positive) is the
+    cross-section normalisation that gates rows and envelopes."""
     _check("K", K)
+    if csn is not None:
+        _check("csn", csn)
+        if csn.numel() != 3 * len(np.atleast_1d(Z)) * K.numel():
+            raise ValueError("csn must hold 3 x n_elements x n_K values")
     Zh = np.ascontiguousarray(np.asarray(Z, dtype=np.float64).ravel())
     Ah = np.ascontiguousarray(np.asarray(A, dtype=np.float64).ravel())
     if Zh.size != Ah.size:
@@ -77,5 +84,6 @@ def dcs_table(Z, A, mass: float, cutoff: float, K: torch.Tensor, fp64: bool = Fa
     with torch.cuda.device(K.device):
         _lib.check(_lib.lib().noa_pumas_dcs_table(Zh.ctypes.data_as(ctypes.c_void_p), Ah.ctypes.data_as(ctypes.c_void_p),
                                                   n_el, float(mass), float(cutoff), _ptr(K), n_K,
-                                                  _ptr(table), _ptr(x), _ptr(env), _ptr(t64), _stream(K)))
+                                                  _ptr(table), _ptr(x), _ptr(env), _ptr(t64),
+                                                  _ptr(csn) if csn is not None else None, _stream(K)))
     return (table, x, env, t64) if fp64 else (table, x, env)

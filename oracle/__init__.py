@@ -252,7 +252,7 @@ def _pumas_sigs(lib) -> None:
     lib.oracle_pumas_n_table_dcs.argtypes = [ctypes.c_double]
     lib.oracle_pumas_n_table_dcs.restype = ctypes.c_int
     lib.oracle_pumas_dcs_table.argtypes = [ctypes.c_int, _D, _D, ctypes.c_double, ctypes.c_double, ctypes.c_int, _D,
-                                           _F, _F, _D, _F, _D, ctypes.c_int]
+                                           _F, _F, _D, _F, _D, _D, ctypes.c_int]
     lib.oracle_pumas_dcs_table.restype = ctypes.c_int
 
 
@@ -293,11 +293,13 @@ def pumas_n_table_dcs(cutoff: float) -> int:
     return lib.oracle_pumas_n_table_dcs(float(cutoff))
 
 
-def pumas_dcs_table(Z, A, mass: float, cutoff: float, K, threads: int = 1, fp64: bool = False):
+def pumas_dcs_table(Z, A, mass: float, cutoff: float, K, threads: int = 1, fp64: bool = False, csn=None):
     """compute_dcs_table over elements (Z, A) and kinetic energies K:
     (dcs [3, n_el, n_K, n_x, 2] float32, x [n_x] float32, x [n_x] float64,
-    envelope exponent [3, n_el, n_K] float32), plus the FP64 table of the
-    same layout when fp64."""
+    envelope [3, n_el, n_K, 2] float32: exponent, maximum), plus the FP64
+    table of the same layout when fp64. csn [3, n_el, n_K]: the cross-section
+    normalisation that gates the envelope maximum (None: every row positive,
+    as the reference harness sets it)."""
     lib = load()
     _pumas_sigs(lib)
     Z = np.ascontiguousarray(Z, dtype=np.float64)
@@ -309,10 +311,12 @@ def pumas_dcs_table(Z, A, mass: float, cutoff: float, K, threads: int = 1, fp64:
     xd = np.zeros(n_x, dtype=np.float64)
     env = np.zeros((3, Z.size, K.size, 2), dtype=np.float32)
     dd = np.zeros(dcs.shape, dtype=np.float64) if fp64 else None
+    cs = None if csn is None else np.ascontiguousarray(np.broadcast_to(csn, (3, Z.size, K.size)), dtype=np.float64)
     assert lib.oracle_pumas_dcs_table(Z.size, _p(Z), _p(A), float(mass), float(cutoff), K.size, _p(K),
                                       dcs.ctypes.data_as(_F), x.ctypes.data_as(_F), _p(xd),
-                                      env.ctypes.data_as(_F), _p(dd) if fp64 else None, threads) == 0
-    return (dcs, x, xd, env[..., 0], dd) if fp64 else (dcs, x, xd, env[..., 0])
+                                      env.ctypes.data_as(_F), _p(dd) if fp64 else None,
+                                      _p(cs) if cs is not None else None, threads) == 0
+    return (dcs, x, xd, env, dd) if fp64 else (dcs, x, xd, env)
 
 
 def load_pumas_ref():

@@ -202,7 +202,7 @@ int oracle_pumas_rule(int n, double* x, double* w) {
 // when not NULL, the table in FP64 (same layout as dcs).
 int oracle_pumas_dcs_table(int n_el, const double* Z, const double* A, double mass, double cutoff, int n_K,
                            const double* K, float* dcs, float* x, double* x_d, float* envelope, double* dcs_d,
-                           int threads) {
+                           const double* csn, int threads) {
     namespace pm = noa_oracle::pumas;
     const int n = pm::n_table_dcs(cutoff);
     pm::x_grid(cutoff, n, x_d);
@@ -217,15 +217,36 @@ int oracle_pumas_dcs_table(int n_el, const double* Z, const double* A, double ma
             const int p = int(r / (int64_t(n_el) * n_K));
             const int e = int((r / n_K) % n_el);
             const int row = int(r % n_K);
+            envelope[2 * r + 1] = 0.f;
+            if (csn && !(csn[r] > 0.)) {  // dcs_tabulate_row's gate (pumas.c:11427-11434); exponent 0 here
+                for (int i = 0; i < 2 * n; i++) {
+                    dcs[r * 2 * n + i] = 0.f;
+                    if (dcs_d) dcs_d[r * 2 * n + i] = 0.;
+                }
+                envelope[2 * r] = 0.f;
+                continue;
+            }
             pm::tabulate_row(p, Z[e], A[e], mass, cutoff, K[row], n, x_d, y, m, dcs + r * 2 * n, envelope + 2 * r,
                              dcs_d ? dcs_d + r * 2 * n : nullptr);
-            envelope[2 * r + 1] = 0.f;
         }
         delete[] y;
         delete[] m;
     }
     for (int p = 0; p < 3; p++)
         for (int e = 0; e < n_el; e++) pm::fill_low_energy_alpha(n_K, envelope + 2 * (int64_t(p) * n_el + e) * n_K, 2);
+    // the envelope maxima, over the finished rows of each process and element
+    // (element 0's search sees the nodes as zeros, as the reference's does)
+    float* zeros = new float[n]();
+#pragma omp parallel for num_threads(threads > 0 ? threads : omp_get_max_threads()) if (threads != 1) \
+    schedule(dynamic, 1)
+    for (int64_t r = 0; r < rows; r++) {
+        const int p = int(r / (int64_t(n_el) * n_K));
+        const int e = int((r / n_K) % n_el);
+        const int row = int(r % n_K);
+        const pm::EnvTable tb{p, Z[e], A[e], mass, K, n_K, dcs + (r - row) * 2 * n, e == 0 ? zeros : x, n};
+        envelope[2 * r + 1] = pm::envelope_max(tb, row, cutoff, envelope[2 * r], csn ? csn[r] : 1.);
+    }
+    delete[] zeros;
     return 0;
 }
 

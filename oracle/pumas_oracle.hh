@@ -6,6 +6,10 @@
 // operation order with glibc's libm, so that its values equal the compiled
 // reference's bit for bit (pinned by tests/test_oracle_pumas.py against
 // tests/golden/pumas_golden.npz, written by oracle/_ref/libpumas_ref.so).
+// The envelope maximum (dcs_tabulate_envelope_row, pumas.c:11509-11679) is
+// restated with its bisection searches (math_find_minimum with algo 0,
+// pumas.c:12172-12232, 12416-12459) and its table interpolation
+// (dcs_evaluate, pumas.c:11307-11398).
 // Only the tests, the smoke check and the benchmark's CPU leg use it.
 #pragma once
 
@@ -563,6 +567,319 @@ inline void fill_low_energy_alpha(int n_K, float* alpha, int stride) {
         }
     }
     for (int row = 0; row < n0; row++) alpha[row * stride] = a;
+}
+
+// ---- the envelope maximum (dcs_tabulate_envelope_row, pumas.c:11509-11679) ----
+
+// One process and element of the table as the envelope search reads it: the
+// rows [n_K][2 n] (float), the nodes xt (float) and the energy grid. The
+// reference reads the nodes from physics->table_DCS_x, which
+// compute_dcs_table writes only after element 0 is done (pumas.c:11775-11783),
+// so the search of element 0 sees zeros there (and takes the model
+// everywhere): callers pass a zero node array for element 0.
+struct EnvTable {
+    int process;
+    double Z, A, mass;
+    const double* K;
+    int n_K;
+    const float* rows;
+    const float* xt;
+    int n;
+};
+
+// table_bracketf (pumas.c:4311-4319), as a loop.
+inline void bracketf(const float* table, float value, int* p1, int* p2) {
+    while (true) {
+        const int i3 = (*p1 + *p2) / 2;
+        if (value >= table[i3])
+            *p1 = i3;
+        else
+            *p2 = i3;
+        if (*p2 - *p1 < 2) break;
+    }
+}
+
+// table_index without a context (pumas.c:4249-4290): the row of K in the
+// energy grid, -1 below it, n_K - 1 at or above its last node.
+inline int table_index(const double* table, int n_K, double value) {
+    const int imax = n_K - 1;
+    if (value < table[0]) return -1;
+    if (value >= table[imax]) return imax;
+    int i1 = 0, i2 = imax;
+    while (true) {  // table_bracket (pumas.c:4299-4306)
+        const int i3 = (i1 + i2) / 2;
+        if (value >= table[i3])
+            i1 = i3;
+        else
+            i2 = i3;
+        if (i2 - i1 < 2) break;
+    }
+    return i1;
+}
+
+// dcs_evaluate without a simulation context (pumas.c:11307-11398) for
+// processes 0-2: the table's cubic Hermite spline in x = ln(q / K), linear in
+// ln K between rows, times the Jacobian K / (K + m); the model itself below
+// 2 q_min (photonuclear) or 4 q_min (pair production), outside the nodes'
+// range, and where one of the four bracketing nodes is empty. (An energy
+// below the grid, where the reference reads before the table, takes the
+// model here; the envelope search never asks for one.)
+inline double dcs_interpolated(const EnvTable& tb, double K, double q) {
+    const double wj = K / (K + tb.mass);
+    double qlow = 0., qmin, qmax;
+    if (tb.process == 2) {
+        photo_range(tb.mass, K, &qmin, &qmax);
+        if ((q < qmin) || (q > qmax)) return 0.;
+        qlow = 2 * qmin;
+    } else {
+        pair_range(tb.Z, tb.mass, K, &qmin, &qmax);
+        if (tb.process == 1)
+            qlow = 4 * qmin;
+        else
+            qmin = 0.;
+        if ((q < qmin) || (q > qmax)) return 0.;
+    }
+    if (q < qlow) return dcs(tb.process, tb.Z, tb.A, tb.mass, K, q) * wj;
+    const int nx = tb.n;
+    const float xmin = tb.xt[0], xmax = tb.xt[nx - 1];
+    const float x = logf((float)(q / K));
+    if ((x < xmin) || (x >= xmax)) return dcs(tb.process, tb.Z, tb.A, tb.mass, K, q) * wj;
+    int ix = 0, tmpi = nx - 1;
+    bracketf(tb.xt, x, &ix, &tmpi);
+    if ((ix < 0) || (ix >= nx - 1)) return dcs(tb.process, tb.Z, tb.A, tb.mass, K, q) * wj;
+    float p00, p01, m00, m01, p10, p11, m10, m11, h0, h1;
+    const int imax = tb.n_K - 1;
+    const double Kmax = tb.K[imax];
+    int i0 = 0;
+    if ((K >= Kmax) || ((i0 = table_index(tb.K, tb.n_K, K)) >= imax)) {
+        const float* data = tb.rows + int64_t(imax) * 2 * nx;
+        p00 = p10 = data[2 * ix];
+        m00 = m10 = data[2 * ix + 1];
+        p01 = p11 = data[2 * ix + 2];
+        m01 = m11 = data[2 * ix + 3];
+        h0 = 1.f;
+        h1 = 0.f;
+    } else {
+        if (i0 < 0) return dcs(tb.process, tb.Z, tb.A, tb.mass, K, q) * wj;
+        const float* data = tb.rows + int64_t(i0) * 2 * nx;
+        p00 = data[2 * ix];
+        m00 = data[2 * ix + 1];
+        p01 = data[2 * ix + 2];
+        m01 = data[2 * ix + 3];
+        data += 2 * nx;
+        p10 = data[2 * ix];
+        m10 = data[2 * ix + 1];
+        p11 = data[2 * ix + 2];
+        m11 = data[2 * ix + 3];
+        const float K0 = (float)tb.K[i0];
+        const float K1 = (float)tb.K[i0 + 1];
+        h1 = logf(((float)K) / K0) / logf(K1 / K0);
+        h0 = 1.f - h1;
+    }
+    if (((p00 == 0.f) && (m00 == 0.f)) || ((p01 == 0.f) && (m01 == 0.f)) || ((p10 == 0.f) && (m10 == 0.f)) ||
+        ((p11 == 0.f) && (m11 == 0.f)))
+        return dcs(tb.process, tb.Z, tb.A, tb.mass, K, q) * wj;
+    const float x0 = tb.xt[ix];
+    const float x1 = tb.xt[ix + 1];
+    const float dx = x1 - x0;
+    m00 *= dx;
+    m01 *= dx;
+    const float t = (x - x0) / dx;
+    const float c02 = -3 * (p00 - p01) - 2 * m00 - m01;
+    const float c03 = 2 * (p00 - p01) + m00 + m01;
+    float r = expf(p00 + t * (m00 + t * (c02 + t * c03))) * h0;
+    if (h1 > 0.f) {
+        m10 *= dx;
+        m11 *= dx;
+        const float c12 = -3 * (p10 - p11) - 2 * m10 - m11;
+        const float c13 = 2 * (p10 - p11) + m10 + m11;
+        r += expf(p10 + t * (m10 + t * (c12 + t * c13))) * h1;
+    }
+    return r * wj;
+}
+
+// dcs_tabulate_envelope_objective (pumas.c:11516-11524): -DCS x^alpha at ln x.
+struct EnvObjective {
+    const EnvTable& tb;
+    double K, alpha;
+    double operator()(double lnx) const {
+        const double x = std::exp(lnx);
+        const double q = x * K;
+        return -dcs_interpolated(tb, K, q) * std::pow(x, alpha);
+    }
+};
+
+// math_find_minimum_bisection (pumas.c:12416-12459).
+template <class F>
+inline int find_minimum_bisection(const F& f, double xa, double xb, double xc, const double* fb_p, double tol,
+                                  int max_iter, double* fopt) {
+    double fb = fb_p ? *fb_p : f(xb);
+    tol *= std::fabs(xc - xa);
+    int i = 0;
+    while ((xc - xa > tol) && (i < max_iter)) {
+        if (xb - xa > xc - xb) {
+            const double x = 0.5 * (xa + xb);
+            const double fi = f(x);
+            if (fi < fb) {
+                xc = xb;
+                xb = x;
+                fb = fi;
+            } else {
+                xa = x;
+            }
+        } else {
+            const double x = 0.5 * (xb + xc);
+            const double fi = f(x);
+            if (fi < fb) {
+                xa = xb;
+                xb = x;
+                fb = fi;
+            } else {
+                xc = x;
+            }
+        }
+        i++;
+    }
+    *fopt = fb;
+    return (i <= max_iter) ? i : -1;
+}
+
+// math_find_minimum with algo 0 (pumas.c:12172-12232): a bisection that
+// brackets a minimum, then the three-point bisection.
+template <class F>
+inline int find_minimum(const F& f, double xa, double xc, const double* fa_p, const double* fc_p, double tol,
+                        int max_iter, double* fopt) {
+    double fa = fa_p ? *fa_p : f(xa);
+    double fc = fc_p ? *fc_p : f(xc);
+    const double atol = tol * std::fabs(xc - xa);
+    double xb = 0, fb = 0;
+    int i;
+    for (i = 0; i < max_iter; i++) {
+        xb = 0.5 * (xa + xc);
+        fb = f(xb);
+        if ((fb < fa) && (fb < fc)) break;
+        if (std::fabs(xc - xa) <= atol) {
+            *fopt = fb;
+            return i;
+        }
+        if (fb < fc) {
+            xc = xb;
+            fc = fb;
+        } else {
+            xa = xb;
+            fa = fb;
+        }
+    }
+    if (i == max_iter) {
+        if (fa < fb) {
+            fb = fa;
+            xb = xa;
+        } else if (fc < fb) {
+            fb = fc;
+            xb = xc;
+        }
+        *fopt = fb;
+        return -1;
+    }
+    const int status = find_minimum_bisection(f, xa, xb, xc, &fb, tol, max_iter - i, fopt);
+    return (status < 0) ? -2 : status + i;
+}
+
+// dcs_tabulate_envelope_row (pumas.c:11526-11679): the maximum of DCS x^alpha
+// over the row's kinematic range above the cutoff, for the row's envelope
+// exponent alpha; 0 where the row's cross-section normalisation cs is not
+// positive or the range is empty.
+inline float envelope_max(const EnvTable& tb, int row, double cutoff, float alpha, double cs) {
+    constexpr double RESOLUTION = 1E-10;
+    if (cs <= 0.) return 0.f;
+    const double K = tb.K[row];
+    double qmin, qmax;
+    if (tb.process == 2) {  // dcs_get_range (pumas.c:11047-11056)
+        photo_range(tb.mass, K, &qmin, &qmax);
+    } else {
+        pair_range(tb.Z, tb.mass, K, &qmin, &qmax);
+        if (tb.process == 0) qmin = 0.;
+    }
+    double xmin = qmin / K;
+    if (xmin < cutoff) xmin = cutoff;
+    xmin = std::log(xmin);
+    double xmax = std::log(qmax / K);
+    if (xmin >= xmax) return 0.f;
+    const EnvObjective f{tb, K, (double)alpha};
+    double f0 = f(xmin);
+    double f1 = f(xmax);
+    if (f0 == 0.) {
+        double x0 = xmin, x1 = xmax;
+        while (x1 - x0 > RESOLUTION) {
+            const double x2 = 0.5 * (x0 + x1);
+            const double f2 = f(x2);
+            if (f2 != 0) {
+                x1 = xmin = x2;
+                f0 = f2;
+            } else {
+                x0 = x2;
+            }
+        }
+    }
+    if (f1 == 0.) {
+        double x0, x1;
+        if (K >= 1E+01) {  // an exponential decrement from the end
+            const double xm = std::exp(xmax);
+            double delta = RESOLUTION, x = xmax, fx = f1;
+            while (xm > delta) {
+                x = std::log(xm - delta);
+                fx = f(x);
+                if (fx != 0.)
+                    break;
+                else
+                    delta += delta;
+            }
+            f1 = fx;
+            x0 = xmax = x;
+            x1 = std::log(xm - delta / 2);
+        } else {
+            x0 = xmin;
+            x1 = xmax;
+        }
+        while (x1 - x0 > RESOLUTION) {
+            const double x2 = 0.5 * (x0 + x1);
+            const double f2 = f(x2);
+            if (f2 != 0) {
+                x0 = xmax = x2;
+                f1 = f2;
+            } else {
+                x1 = x2;
+            }
+        }
+    }
+    const double xm = 0.5 * (xmin + xmax);
+    const double fm = f(xm);
+    double fopt0 = 0., fopt1 = 0.;
+    find_minimum(f, xmin, xm, &f0, &fm, RESOLUTION, 100, &fopt0);
+    find_minimum(f, xm, xmax, &fm, &f1, RESOLUTION, 100, &fopt1);
+    if ((tb.process == 1) && (K >= 1E+07)) {
+        // a sharp peak next to xmax: a logarithmic scan for a bracketing
+        constexpr int N_SCAN = 7;
+        double xs[N_SCAN], fs[N_SCAN], fmin = 0.;
+        int imin = -1;
+        for (int i = 0; i < N_SCAN; i++) {
+            xs[i] = xmax + std::log(1. - std::pow(10, -(i + 1)));
+            fs[i] = f(xs[i]);
+            if (fs[i] < fmin) {
+                imin = i;
+                fmin = fs[i];
+            }
+        }
+        if ((imin > 0) && (imin < N_SCAN - 1)) {
+            double fopt2 = 0.;
+            find_minimum_bisection(f, xs[imin - 1], xs[imin], xs[imin + 1], fs + imin, RESOLUTION, 100, &fopt2);
+            if (fopt2 < fopt1) fopt1 = fopt2;
+        }
+    }
+    double fopt = (fopt0 < fopt1) ? fopt0 : fopt1;
+    if (f0 < fopt) fopt = f0;
+    if (f1 < fopt) fopt = f1;
+    return (float)(-fopt);
 }
 
 }  // namespace pumas

@@ -10,6 +10,8 @@ Contents:
   table_<set>_dcs                   compute_dcs_table's rows [3, n_el, n_K, n_x, 2] (float32)
   table_<set>_x                     its nodes table_DCS_x [n_x] (float32)
   table_<set>_alpha                 the envelope exponents [3, n_el, n_K] (float32)
+  table_<set>_envmax                the envelope maxima [3, n_el, n_K] (float32; the cross-section
+                                    normalisation set to 1, so every row runs the search)
   point_{Z,A,K,q}, point_dcs        the model DCS (m^2/kg) of processes 0, 1, 2 at (K, q)
                                     pairs of four elements, muon: [3, n] (float64)
 Set "muon": 6 elements, 61 kinetic energies 1e-3..1e7 GeV, cutoff 0.05;
@@ -69,7 +71,8 @@ def main() -> None:
         out.update({f"table_{name}_Z": np.array(s["Z"]), f"table_{name}_A": np.array(s["A"]),
                     f"table_{name}_K": s["K"], f"table_{name}_mass": np.array(s["mass"]),
                     f"table_{name}_cutoff": np.array(s["cutoff"]), f"table_{name}_dcs": dcs,
-                    f"table_{name}_x": x, f"table_{name}_alpha": env[..., 0]})
+                    f"table_{name}_x": x, f"table_{name}_alpha": env[..., 0],
+                    f"table_{name}_envmax": env[..., 1]})
     rng = np.random.default_rng(2024)
     Zs, As, Ks, qs = [], [], [], []
     for Z, A in ((1.0, 1.00794), (11.0, 22.0), (26.0, 55.845), (92.0, 238.02891)):

@@ -8,7 +8,8 @@ DCS and the logarithms of the FP64 table within 1e-10 relative with the same
 zero pattern; the slopes within what the logarithms' deviation allows over
 the smallest node spacing; the float table, PUMAS's own format, within 2
 float ulps, with the same zero pattern; the nodes exactly; the envelope
-exponents within 1 float ulp.
+exponents within 1 float ulp; the envelope maxima with the same NaN and zero
+patterns and within the relative error that the table's ulps allow.
 """
 import os
 
@@ -60,7 +61,31 @@ def test_table_matches_reference_golden(name, device):
     a = env[..., 0]
     assert np.array_equal(a == 0, s["alpha"] == 0)
     assert _ulps32(a, s["alpha"]).max() <= 1
-    assert np.all(env[..., 1] == 0)
+    _check_envmax(env[..., 1], s["envmax"], f"{name} envelope maximum", s["dcs"])
+
+
+def _check_envmax(got, exp, what, table):
+    """The envelope maximum: NaN where the reference's is (rows below the
+    kinematic threshold with the normalisation forced to 1), zero where it is
+    zero, and elsewhere within what the table's own tolerance allows. The
+    search finds the same extremum, where the DCS is exp of the interpolated
+    ln DCS: the table's ln values are within 2 float ulps of the reference's,
+    i.e. within 2 ulp(max |ln DCS| of the row) absolute, so the maximum is
+    within that much relative (3 of those ulps are allowed, plus 2 float ulps
+    of the final rounding)."""
+    assert np.array_equal(np.isnan(got), np.isnan(exp)), (what, "NaN pattern")
+    ok = ~np.isnan(exp)
+    assert np.array_equal(got[ok] == 0, exp[ok] == 0), (what, "zero pattern")
+    ymax = np.abs(table[..., 0]).max(axis=-1).astype(np.float64)  # per row
+    ulp_y = np.exp2(np.floor(np.log2(np.maximum(ymax, 1e-30))) - 23)
+    bound = 3 * ulp_y + 2.0 ** -22
+    with np.errstate(invalid="ignore", divide="ignore"):
+        rel = np.abs(got.astype(np.float64) / exp.astype(np.float64) - 1)
+    rel = np.where(ok & (exp != 0), rel, 0.0)
+    u = _ulps32(got[ok], exp[ok])
+    print(f"{what}: max {int(u.max())} ulps, max rel {rel.max():.2e}, {int((u > 0).sum())} of {u.size} differ",
+          flush=True)
+    assert np.all(rel <= bound), (what, float(rel.max()), np.argwhere(rel > bound)[:4])
 
 
 @pytest.mark.parametrize("mass,cutoff", [(MUON, 0.05), (1.77682, 1e-3), (MUON, 0.7)])
@@ -71,7 +96,8 @@ def test_fp64_table_matches_oracle(mass, cutoff, device):
     K = np.sort(10.0 ** rng.uniform(-3, 8, 50))
     print(f"pumas fp64 table mass={mass} cutoff={cutoff}", flush=True)
     table, x, env, t64 = _gpu_table(Z, A, mass, cutoff, K, device, fp64=True)
-    dcs, xf, _, alpha, d64 = oracle.pumas_dcs_table(Z, A, mass, cutoff, K, threads=0, fp64=True)
+    dcs, xf, _, oenv, d64 = oracle.pumas_dcs_table(Z, A, mass, cutoff, K, threads=0, fp64=True)
+    alpha = oenv[..., 0]
     assert np.array_equal(x, xf)
     c = assert_parity(t64[..., 0], d64[..., 0], 1e-10, "ln DCS")
     # a slope is a quotient of differences of the row's logarithms over node
@@ -88,6 +114,25 @@ def test_fp64_table_matches_oracle(mass, cutoff, device):
     assert np.all(err <= bound), (float(err.max()), float(bound[err > bound].min()))
     _check_table(table, dcs, "float table")
     assert np.array_equal(env[..., 0] == 0, alpha == 0) and _ulps32(env[..., 0], alpha).max() <= 1
+    _check_envmax(env[..., 1], oenv[..., 1], "envelope maximum", dcs)
+
+
+def test_cross_section_gate_matches_oracle(device):
+    """A normalisation table with empty rows: those rows are zeros with no
+    envelope, the others as without it (pumas.c:11427-11434, 11532-11536)."""
+    rng = np.random.default_rng(8)
+    Z = np.array([6.0, 26.0, 82.0])
+    A = Z * rng.uniform(2.0, 2.6, Z.size)
+    K = np.sort(10.0 ** rng.uniform(-2, 7, 24))
+    csn = np.where(rng.uniform(size=(3, Z.size, K.size)) < 0.3, 0.0, 1.0)
+    Kd = torch.from_numpy(K).to(device)
+    table, x, env = (t.cpu().numpy() for t in pumas.dcs_table(Z, A, MUON, 0.05, Kd,
+                                                            csn=torch.from_numpy(csn).to(device)))
+    dcs, xf, _, oenv = oracle.pumas_dcs_table(Z, A, MUON, 0.05, K, threads=0, csn=csn)
+    assert np.all(table[csn == 0] == 0) and np.all(env[csn == 0][:, 1] == 0)
+    _check_table(table, dcs, "gated table")
+    assert np.array_equal(env[..., 0] == 0, oenv[..., 0] == 0) and _ulps32(env[..., 0], oenv[..., 0]).max() <= 1
+    _check_envmax(env[..., 1], oenv[..., 1], "gated envelope maximum", dcs)
 
 
 def test_model_dcs_matches_reference_golden(device):

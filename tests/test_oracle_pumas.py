@@ -38,11 +38,15 @@ def _bits(a):
 @pytest.mark.parametrize("name", ["muon", "tau"])
 def test_table_bitwise_equals_reference_golden(golden, name):
     g = {k[len(f"table_{name}_"):]: golden[k] for k in golden.files if k.startswith(f"table_{name}_")}
-    dcs, x, xd, alpha = oracle.pumas_dcs_table(g["Z"], g["A"], float(g["mass"]), float(g["cutoff"]), g["K"],
-                                               threads=0)
+    dcs, x, xd, env = oracle.pumas_dcs_table(g["Z"], g["A"], float(g["mass"]), float(g["cutoff"]), g["K"],
+                                             threads=0)
     assert np.array_equal(_bits(x), _bits(g["x"]))
     assert np.array_equal(_bits(dcs), _bits(g["dcs"]))
-    assert np.array_equal(_bits(alpha), _bits(g["alpha"]))
+    assert np.array_equal(_bits(env[..., 0]), _bits(g["alpha"]))
+    # the envelope maxima, NaNs included (rows below the kinematic threshold,
+    # which the cross-section normalisation of 1 lets through), bit for bit
+    assert np.array_equal(_bits(env[..., 1]), _bits(g["envmax"]))
+    assert (g["envmax"] > 0).mean() > 0.7 and np.isnan(g["envmax"]).any()
     # the rows hold both zero and positive-DCS regions
     assert (g["dcs"][..., 0] == 0).any() and (g["dcs"][..., 0] != 0).any()
 
@@ -69,11 +73,11 @@ def test_table_bitwise_equals_reference_build(mass, cutoff):
     Z = np.array([2.0, 8.0, 29.0, 74.0])
     A = Z * rng.uniform(2.0, 2.6, Z.size)
     K = np.sort(10.0 ** rng.uniform(-3, 8, 37))
-    dcs, x, _, alpha = oracle.pumas_dcs_table(Z, A, mass, cutoff, K, threads=0)
+    dcs, x, _, env = oracle.pumas_dcs_table(Z, A, mass, cutoff, K, threads=0)
     rd, rx, renv = oracle.ref_pumas_dcs_table(Z, A, mass, cutoff, K)
     assert np.array_equal(_bits(x), _bits(rx))
     assert np.array_equal(_bits(dcs), _bits(rd))
-    assert np.array_equal(_bits(alpha), _bits(renv[..., 0]))
+    assert np.array_equal(_bits(env), _bits(renv))
 
 
 def test_model_dcs_bitwise_equals_reference_build():
