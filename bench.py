@@ -42,6 +42,7 @@ def parse() -> argparse.Namespace:
     ap.add_argument("--n", type=int, default=1 << 24)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-extras", action="store_true", help="only the FP64 headline")
+    ap.add_argument("--mixed-only", action="store_true", help="the FP64 headline and the mixed form only")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="after the run, one plain device copy of BYTES (scales rocprof FETCH/WRITE_SIZE)")
     return ap.parse_args()
@@ -369,6 +370,8 @@ def main() -> None:
                                                        read_counters("dcs_eval_f32mixed_2e24_valu.txt"),
                                                        flops_pair, "F32")}
         del o32
+        result["forms"] = forms  # filled further below unless --mixed-only
+    if not args.no_extras and not args.mixed_only:
         # per-process rates: one output requested, the other processes skipped
         per = {}
         for pi, name in enumerate(("bremsstrahlung", "pair_production", "photonuclear", "ionisation")):
@@ -430,7 +433,6 @@ def main() -> None:
                                                      "dispatch between the launches included)"}
         del outs_b
         forms["pumas_dcs_table"] = pumas_table_entry(dev, stream, max(3, args.steps // 10), not args.no_cpu)
-        result["forms"] = forms
 
     if args.calibrate_copy:
         src = torch.ones(args.calibrate_copy // 8, dtype=torch.float64, device=dev)
