@@ -21,7 +21,8 @@
 //                    R and theta are carried as fractions, so that 1 / (1 + R)
 //                    is D_R D_theta / S, and one reciprocal of Q^2 (Q^2 + m0^2)
 //                    g_P g_A S replaces the reference's eight divisions; log10
-//                    as log / ln 10.
+//                    as log / ln 10; its sums of products are FMAs (those
+//                    whose terms do not cancel).
 // Inside the node loops, log_pos / exp_fin (no special-argument paths) take
 // the arguments that are positive normal / finite by construction for total
 // energies up to 1e30 GeV: 1 + 1/xi, 3 + xi, the screening arguments, Q^2's
@@ -452,17 +453,25 @@ struct F64N {
         }
         const double inv_2Mq = inv_q * (0.5 / M);
         const double inv_l2 = 1. / ALLM_LAMBDA2;
+        // per-pair terms of the node expressions' fused forms (fast path):
+        // x_P = Q^2 / d_P + 49.457 / d_P, the same for x_R, y^2 + Q^2 / E^2,
+        // 1 - y and 1 / (4 E^2)
+        const double cxP = 49.457 * inv_dP, cxR = 0.15052 * inv_dR;
+        const double yy = y * y, q25 = 0.25 * inv_EE;
 
         double ds = 0.;
 #pragma unroll NU
         for (int j = 0; j < 9; j++) {
-            const double Q2 = nexp<Safe>(pQ2c + hd * GL9_X[j], t);
+            // the fast path fuses products into their sums (FMAs with at most
+            // one literal operand: gfx950's VOP3 form takes one scalar and no
+            // literal, so a second literal would cost two moves)
+            const double Q2 = nexp<Safe>(Safe ? pQ2c + hd * GL9_X[j] : fma(hd, GL9_X[j], pQ2c), t);
             // dcs_photonuclear_d2 (dcs.hh:335-355)
             const double x = Q2 * inv_2Mq;
             // dcs_photonuclear_f2_allm (dcs.hh:261-307)
             const double tl = nlog<Safe>(nlog<Safe>((Q2 + ALLM_Q02) * inv_l2, t) * c.ph_inv_L0, t);
-            const double xP = (Q2 + 49.457) * inv_dP;
-            const double xR = (Q2 + 0.15052) * inv_dR;
+            const double xP = Safe ? (Q2 + 49.457) * inv_dP : fma(Q2, inv_dP, cxP);
+            const double xR = Safe ? (Q2 + 0.15052) * inv_dR : fma(Q2, inv_dR, cxR);
             // t -> 0 (Q^2 -> 0) gives log t = -inf and t^c = 0; clamped at -745
             // every t^c still rounds to 0 against the constants it is added to
             const double lnt = fmax(fm::log(tl, t), -745.0);
@@ -479,8 +488,8 @@ struct F64N {
                 q2 = fmax(Q2, 0.3);
             }
             const double lg = nlog<Safe>(q2 * 25.0, t);  // q2 / 0.04
-            const double den = 0.09 + q2 * q2;
-            const double Dth = (1.0 + q2) * (0.015625 + x * x);
+            const double den = Safe ? 0.09 + q2 * q2 : fma(q2, q2, 0.09);
+            const double Dth = (1.0 + q2) * (Safe ? 0.015625 + x * x : fma(x, x, 0.015625));
             const double Qm = Q2 + 0.31985;
             const double gPA = gP * gA;
             double inv_Q2, inv_Qm, rG, inv_1R;  // 1 / Q^2, 1 / (Q^2 + m0^2), 1 / (g_P g_A), 1 / (1 + R)
@@ -495,8 +504,8 @@ struct F64N {
                 // theta = (D_th + 12 c q2) / D_th and R = N_R / D_R as fractions,
                 // so 1 / (1 + R) = D_R D_th / S; the four over one reciprocal
                 const double DR = lg * q2 * den;
-                const double S = Dth * (DR + lg * (0.5747 * den - 0.3534 * q2)) +
-                                 0.635 * q2 * den * (Dth + (12.0 * 0.015625) * q2);
+                const double S = fma(Dth, fma(lg, fma(0.5747, den, -0.3534 * q2), DR),
+                                     0.635 * q2 * den * fma(12.0 * 0.015625, q2, Dth));
                 const double QQm = Q2 * Qm, gS = gPA * S;
                 const double r = fm::recip_pos(QQm * gS);
                 const double rq = gS * r, rg = QQm * r;
@@ -505,11 +514,12 @@ struct F64N {
                 rG = S * rg;
                 inv_1R = DR * Dth * (gPA * rg);
             }
-            const double cP = 0.28067 + (0.28067 - 0.22291) * (gA * rG - 1.0);
-            const double aP = -0.0808 + (-0.0808 - -0.44812) * (gP * rG - 1.0);
+            const double cP = 0.28067 + (0.28067 - 0.22291) * (Safe ? gA * rG - 1.0 : fma(gA, rG, -1.0));
+            const double aP = -0.0808 + (-0.0808 - -0.44812) * (Safe ? gP * rG - 1.0 : fma(gP, rG, -1.0));
             const double l1x = nlog<Safe>(1. - x, t);
-            const double F2P = cP * nexp<Safe>(aP * nlog<Safe>(xP, t) + bP * l1x, t);
-            const double F2R = cR * nexp<Safe>(aR * nlog<Safe>(xR, t) + bR * l1x, t);
+            const double lxP = nlog<Safe>(xP, t), lxR = nlog<Safe>(xR, t);
+            const double F2P = cP * nexp<Safe>(Safe ? aP * lxP + bP * l1x : fma(aP, lxP, bP * l1x), t);
+            const double F2R = cR * nexp<Safe>(Safe ? aR * lxR + bR * l1x : fma(aR, lxR, bR * l1x), t);
             const double F2p = Q2 * inv_Qm * (F2P + F2R);
             // dcs_photonuclear_f2a_drss (dcs.hh:310-319)
             double a = 1.0;
@@ -517,12 +527,22 @@ struct F64N {
                 a = c.ph_a_low;
             else if (x < 0.04)
                 a = nexp<Safe>((0.069 * (nlog<Safe>(x, t) * INV_LN10) + 0.097) * c.ph_logA, t);
-            const double F2A = c.ph_halfA * a * (2.0 + x * (-1.85 + x * (2.45 + x * (-2.35 + x)))) * F2p;
-            const double dds =
-                (1. - y + 0.5 * (1. - c.ph_2m2 * inv_Q2) * (y * y + Q2 * inv_EE) * inv_1R) * (inv_Q2 * inv_Q2) -
-                0.25 * inv_EE * inv_Q2;
+            double poly, dds;
+            if constexpr (Safe) {
+                poly = 2.0 + x * (-1.85 + x * (2.45 + x * (-2.35 + x)));
+                dds = (1. - y + 0.5 * (1. - c.ph_2m2 * inv_Q2) * (y * y + Q2 * inv_EE) * inv_1R) * (inv_Q2 * inv_Q2) -
+                      0.25 * inv_EE * inv_Q2;
+            } else {
+                poly = fma(x, fma(x, fma(x, x - 2.35, 2.45), -1.85), 2.0);
+                const double w = 0.5 * fma(-c.ph_2m2, inv_Q2, 1.) * fma(Q2, inv_EE, yy);
+                dds = fma(fma(w, inv_1R, omy), inv_Q2 * inv_Q2, -(q25 * inv_Q2));
+            }
+            const double F2A = c.ph_halfA * a * poly * F2p;
             const double d2 = 2.603096E-35 * F2A * dds * inv_q;
-            ds += d2 * Q2 * 1. * GL9_W[j];
+            if constexpr (Safe)
+                ds += d2 * Q2 * 1. * GL9_W[j];
+            else
+                ds = fma(d2 * Q2, GL9_W[j], ds);
         }
         return (ds < 0.) ? 0. : dv<!Rn>(0.5 * ds * dpQ2 * 1E+03 * AVOGADRO_NUMBER * (c.mass + K), c.A, c.r_A);
     }
