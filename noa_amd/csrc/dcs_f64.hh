@@ -21,8 +21,8 @@
 //                    R and theta are carried as fractions, so that 1 / (1 + R)
 //                    is D_R D_theta / S, and one reciprocal of Q^2 (Q^2 + m0^2)
 //                    g_P g_A S replaces the reference's eight divisions; log10
-//                    as log / ln 10; its sums of products are FMAs (those
-//                    whose terms do not cancel).
+//                    as log / ln 10; its sums of products are FMAs where
+//                    they save an instruction.
 // Inside the node loops, log_pos / exp_fin (no special-argument paths) take
 // the arguments that are positive normal / finite by construction for total
 // energies up to 1e30 GeV: 1 + 1/xi, 3 + xi, the screening arguments, Q^2's
