@@ -269,7 +269,6 @@ struct F64N {
         const double Nm0 = 4. + 3. * beta, Nm1 = 1. + 3. * beta;  // N_mu = 4 + rho2 + 3 beta (1 + rho2)
         const double De0 = -4. * beta, De1 = 2. * beta - 1., De2 = 2. * Nm1;  // D_e = De2 L - rho2 - 2 beta (2 - rho2)
         const double Dm2 = 1.5 + 2. * beta;
-        const double mtw = -tmin;
 
         double I = 0.;
 #pragma unroll NU
@@ -303,17 +302,25 @@ struct F64N {
                 Ne = fma(rho2, Ne1, Ne0);
                 Nm = fma(rho2, Nm1, Nm0);
             }
-            double dr, Ye, Ymu;  // dr = (rho21 - beta) / (1 + xi), shared by B_e and B_mu
+            // dr = (rho21 - beta) / (1 + xi), shared by B_e and B_mu; x_e = (1 + xi)
+            // (1 + Y_e), x_mu = (1 + xi) (1 + Y_mu) with Y_e = N_e / D_e, Y_mu = N_mu / D_mu
+            double dr, xe, xmu;
             if constexpr (Safe) {
                 dr = (rho21 - beta) / xi1;
-                Ye = Ne / De;
-                Ymu = Nm / Dm;
+                xe = xi1 * (1. + Ne / De);
+                xmu = xi1 * (1. + Nm / Dm);
             } else {  // one reciprocal for the three (D_e, D_mu > 1)
                 const double DD = De * Dm;
                 const double r_y = fm::recip_pos(DD * xi1);
                 dr = (rho21 - beta) * (DD * r_y);
-                Ye = Ne * ((Dm * xi1) * r_y);
-                Ymu = Nm * ((De * xi1) * r_y);
+                if constexpr (Rn) {
+                    const double u = xi1 * (xi1 * r_y);  // (1 + xi) / (D_e D_mu)
+                    xe = fma(Ne * Dm, u, xi1);
+                    xmu = fma(Nm * De, u, xi1);
+                } else {  // (the mixed form's recomputation: fewer registers live)
+                    xe = xi1 * (1. + Ne * ((Dm * xi1) * r_y));
+                    xmu = xi1 * (1. + Nm * ((De * xi1) * r_y));
+                }
             }
             // B_e and B_mu keep the reference's order: near xi = 1e3 and 1e-3
             // their terms cancel, and any reordering moves the result by up to
@@ -328,42 +335,46 @@ struct F64N {
                 Bmu = 0.5 * xi * (5. - rho2 + beta * (3. + rho2));
             else
                 Bmu = (opr2 * c15b - xi_i * c2b * rho21) * nlog<Safe>(1. + xi, t) + xi * dr + c2b * rho21;
-            const double xe = xi1 * (1. + Ye);
-            const double xmu = xi1 * (1. + Ymu);
             // q + (c_L / rho21) x = (q rho21 + c_L x) / rho21
             const double qr = q * rho21;
-            double Le, Lmu;
             if constexpr (Safe) {
                 const double ge = qr + c.p_cL * xe;
-                Le = fm::log(c.p_AZ13 * qr * sqrt(xe / ((1. + c.p_cLe * xe) * (ge * ge))), t);
-                Lmu = fm::log(c.p_rAZ13 * qr / (c.p_15Z13 * (qr + c.p_cL * xmu)), t);
-            } else {  // one reciprocal for the two screening denominators, no square root
+                const double Le = fm::log(c.p_AZ13 * qr * sqrt(xe / ((1. + c.p_cLe * xe) * (ge * ge))), t);
+                const double Lmu = fm::log(c.p_rAZ13 * qr / (c.p_15Z13 * (qr + c.p_cL * xmu)), t);
+                double Phi_e = Be * Le;
+                double Phi_mu = Bmu * Lmu;
+                if (Phi_e < 0.) Phi_e = 0.;
+                if (Phi_mu < 0.) Phi_mu = 0.;
+                const double f = -(Phi_e + Phi_mu * c.p_inv_r2) * (1. - rho) * tmin;
+                I += f * GL8_W[j];
+            } else {
+                // one reciprocal for the two screening denominators, no square
+                // root; L_e2 = 2 L_e, so Phi_e2 = 2 Phi_e exactly, and the node
+                // sums 2 (Phi_e + Phi_mu / r^2) (1 - rho) w_j, scaled by
+                // -tmin / 2 after the loop (in the mixed form's recomputation,
+                // which keeps fewer registers live, each node's sum is scaled
+                // by -tmin and Phi_e is formed directly). 1 - rho as the reference rounds it
+                // (not eps: they differ by ~1e-16 absolute, which is ~1e-11
+                // relative where eps ~ 1e-5)
                 const double ge = fma(c.p_cL, xe, qr);
                 const double Dle = fma(c.p_cLe, xe, 1.) * (ge * ge);
                 const double Dlm = c.p_15Z13 * fma(c.p_cL, xmu, qr);
                 const double r_l = fm::recip_pos(Dle * Dlm);
-                Le = 0.5 * fm::log_pos(c.p_AZ13_sq * (qr * qr) * xe * (Dlm * r_l), t);
-                Lmu = fm::log_pos(c.p_rAZ13 * qr * (Dle * r_l), t);
-            }
-            double Phi_e = Be * Le;
-            double Phi_mu = Bmu * Lmu;
-            if constexpr (Safe) {
-                if (Phi_e < 0.) Phi_e = 0.;
-                if (Phi_mu < 0.) Phi_mu = 0.;
-            } else {  // one v_max each; a NaN here comes from a NaN q, which
-                // the (E - q) / q factor of the result carries anyway
-                Phi_e = fmax(Phi_e, 0.);
-                Phi_mu = fmax(Phi_mu, 0.);
-            }
-            if constexpr (Safe) {
-                const double f = -(Phi_e + Phi_mu * c.p_inv_r2) * (1. - rho) * tmin;
-                I += f * GL8_W[j];
-            } else {  // f = -(Phi_e + Phi_mu / r^2) (1 - rho) tmin; 1 - rho as the
-                // reference rounds it (not eps: they differ by ~1e-16 absolute,
-                // which is ~1e-11 relative where eps ~ 1e-5)
-                I = fma(fma(Phi_mu, c.p_inv_r2, Phi_e), (1. - rho) * (mtw * GL8_W[j]), I);
+                const double Le2 = fm::log_pos(c.p_AZ13_sq * (qr * qr) * xe * (Dlm * r_l), t);
+                const double Lmu = fm::log_pos(c.p_rAZ13 * qr * (Dle * r_l), t);
+                // one v_max each; a NaN here comes from a NaN q, which the
+                // (E - q) / q factor of the result carries anyway
+                const double Phi_mu = fmax(Bmu * Lmu, 0.);
+                if constexpr (Rn) {
+                    const double Phi_e2 = fmax(Be * Le2, 0.);
+                    I = fma(fma(Phi_mu, 2. * c.p_inv_r2, Phi_e2), (1. - rho) * GL8_W[j], I);
+                } else {
+                    const double Phi_e = fmax(Be * (0.5 * Le2), 0.);
+                    I = fma(fma(Phi_mu, c.p_inv_r2, Phi_e), (1. - rho) * (-tmin * GL8_W[j]), I);
+                }
             }
         }
+        if constexpr (Rn) I *= -0.5 * tmin;
 
         double zeta;
         if (gamma <= 35.) {
@@ -473,8 +484,12 @@ struct F64N {
             const double xP = Safe ? (Q2 + 49.457) * inv_dP : fma(Q2, inv_dP, cxP);
             const double xR = Safe ? (Q2 + 0.15052) * inv_dR : fma(Q2, inv_dR, cxR);
             // t -> 0 (Q^2 -> 0) gives log t = -inf and t^c = 0; clamped at -745
-            // every t^c still rounds to 0 against the constants it is added to
-            const double lnt = fmax(fm::log(tl, t), -745.0);
+            // every t^c still rounds to 0 against the constants it is added to.
+            // The FP64 kernel's fast path clamps t to the smallest normal instead (zero,
+            // negative and NaN t included): log t >= -708.4 there, where every
+            // t^c rounds to 0 against those constants too, so the results are
+            // the same and the log needs no special-argument path
+            const double lnt = Rn ? fm::log_pos(fmax(tl, 0x1p-1022), t) : fmax(fm::log(tl, t), -745.0);
             const double gP = 1.0 + nexp<Safe>(2.1979 * lnt, t), gA = 1.0 + nexp<Safe>(1.1709 * lnt, t);
             const double bP = 0.36292 + 1.8917 * nexp<Safe>(1.8439 * lnt, t);
             const double cR = 0.80107 + 0.97307 * nexp<Safe>(3.4942 * lnt, t);
@@ -534,16 +549,25 @@ struct F64N {
                       0.25 * inv_EE * inv_Q2;
             } else {
                 poly = fma(x, fma(x, fma(x, x - 2.35, 2.45), -1.85), 2.0);
-                const double w = 0.5 * fma(-c.ph_2m2, inv_Q2, 1.) * fma(Q2, inv_EE, yy);
+                // 0.5 (1 - 2 m^2 / Q^2): one FMA in the FP64 kernel (the same
+                // value: m^2 = 2 m^2 / 2 exactly), as written in the mixed
+                // form's recomputation, which then needs one constant less
+                const double w = (Rn ? fma(-c.ph_m2, inv_Q2, 0.5) : 0.5 * fma(-c.ph_2m2, inv_Q2, 1.)) *
+                                 fma(Q2, inv_EE, yy);
                 dds = fma(fma(w, inv_1R, omy), inv_Q2 * inv_Q2, -(q25 * inv_Q2));
             }
-            const double F2A = c.ph_halfA * a * poly * F2p;
-            const double d2 = 2.603096E-35 * F2A * dds * inv_q;
-            if constexpr (Safe)
-                ds += d2 * Q2 * 1. * GL9_W[j];
-            else
-                ds = fma(d2 * Q2, GL9_W[j], ds);
+            if constexpr (Rn) {  // the per-pair factor A / 2 * 2.603096e-35 / q after the loop
+                ds = fma((a * poly) * F2p * (dds * Q2), GL9_W[j], ds);
+            } else {
+                const double F2A = c.ph_halfA * a * poly * F2p;
+                const double d2 = 2.603096E-35 * F2A * dds * inv_q;
+                if constexpr (Safe)
+                    ds += d2 * Q2 * 1. * GL9_W[j];
+                else
+                    ds = fma(d2 * Q2, GL9_W[j], ds);
+            }
         }
+        if constexpr (Rn) ds *= c.ph_halfA * 2.603096E-35 * inv_q;
         return (ds < 0.) ? 0. : dv<!Rn>(0.5 * ds * dpQ2 * 1E+03 * AVOGADRO_NUMBER * (c.mass + K), c.A, c.r_A);
     }
 
