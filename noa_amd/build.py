@@ -103,13 +103,18 @@ def build_extension(verbose: bool = False) -> str:
 
 def build_test_kernels() -> str:
     """tests/native/libfm_dev.so: the device log / exp / reciprocal of
-    csrc/fastmath.hh over arrays, for the GPU accuracy tests (not part of the
+    csrc/fastmath.hh over arrays, for the GPU accuracy tests; and
+    tests/native/libclock_probe.so: the FP64 tile kernel with per-workgroup
+    clock counters, for the GPU measurement test (neither is part of the
     product library)."""
     native = os.path.join(ROOT, "tests", "native")
-    out = os.path.join(native, "libfm_dev.so")
-    _run([HIPCC, *HIP_FLAGS, "-shared", "-o", out + ".tmp", os.path.join(native, "fastmath_dev.hip")])
-    os.replace(out + ".tmp", out)
-    return out
+    outs = []
+    for src, name in (("fastmath_dev.hip", "libfm_dev.so"), ("clock_probe.hip", "libclock_probe.so")):
+        out = os.path.join(native, name)
+        _run([HIPCC, *HIP_FLAGS, "-shared", "-o", out + ".tmp", os.path.join(native, src)])
+        os.replace(out + ".tmp", out)
+        outs.append(out)
+    return ", ".join(outs)
 
 
 def build_oracle(with_reference: bool = True) -> None:
