@@ -463,6 +463,7 @@ struct F64N {
             inv_dR = dP * (qE * r_pair);
         }
         const double inv_2Mq = inv_q * (0.5 / M);
+        const double l_inv_2Mq = Rn ? fm::log_pos(inv_2Mq, t) : 0.;  // 1 / (2 M q) is positive normal here
         const double inv_l2 = 1. / ALLM_LAMBDA2;
         // per-pair terms of the node expressions' fused forms (fast path):
         // x_P = Q^2 / d_P + 49.457 / d_P, the same for x_R, y^2 + Q^2 / E^2,
@@ -476,7 +477,8 @@ struct F64N {
             // the fast path fuses products into their sums (FMAs with at most
             // one literal operand: gfx950's VOP3 form takes one scalar and no
             // literal, so a second literal would cost two moves)
-            const double Q2 = nexp<Safe>(Safe ? pQ2c + hd * GL9_X[j] : fma(hd, GL9_X[j], pQ2c), t);
+            const double lQ2 = Safe ? pQ2c + hd * GL9_X[j] : fma(hd, GL9_X[j], pQ2c);  // log Q^2 of the node
+            const double Q2 = nexp<Safe>(lQ2, t);
             // dcs_photonuclear_d2 (dcs.hh:335-355)
             const double x = Q2 * inv_2Mq;
             // dcs_photonuclear_f2_allm (dcs.hh:261-307)
@@ -502,7 +504,11 @@ struct F64N {
             } else {  // a NaN Q^2 (NaN q) still reaches the result through Q^2 itself
                 q2 = fmax(Q2, 0.3);
             }
-            const double lg = nlog<Safe>(q2 * 25.0, t);  // q2 / 0.04
+            // log(q2 / 0.04); in the FP64 kernel from the node's log Q^2 (the
+            // exponent of Q^2 itself, so one log less per node): ln 25 + log Q^2
+            // above the clamp, ln 7.5 at it
+            const double lg = Rn ? ((Q2 >= 0.3) ? lQ2 + 3.2188758248682006 : 2.0149030205422647)
+                                 : nlog<Safe>(q2 * 25.0, t);
             const double den = Safe ? 0.09 + q2 * q2 : fma(q2, q2, 0.09);
             const double Dth = (1.0 + q2) * (Safe ? 0.015625 + x * x : fma(x, x, 0.015625));
             const double Qm = Q2 + 0.31985;
@@ -540,8 +546,8 @@ struct F64N {
             double a = 1.0;
             if (x < 0.0014)
                 a = c.ph_a_low;
-            else if (x < 0.04)
-                a = nexp<Safe>((0.069 * (nlog<Safe>(x, t) * INV_LN10) + 0.097) * c.ph_logA, t);
+            else if (x < 0.04)  // log x = log Q^2 - log(2 M q) in the FP64 kernel
+                a = nexp<Safe>((0.069 * ((Rn ? lQ2 + l_inv_2Mq : nlog<Safe>(x, t)) * INV_LN10) + 0.097) * c.ph_logA, t);
             double poly, dds;
             if constexpr (Safe) {
                 poly = 2.0 + x * (-1.85 + x * (2.45 + x * (-2.35 + x)));
