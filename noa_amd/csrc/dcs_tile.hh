@@ -87,27 +87,20 @@ __device__ __forceinline__ void load_fm_tables(double* block) {
     for (int k = threadIdx.x; k < fm::TABLE_DOUBLES; k += WG) block[k] = fm::table_entry(k);
 }
 
-// Evaluates pairs [base, end) (end - base <= TILE) of one element with the
-// log/exp tables `t` in LDS. Must be called by all WG threads of the
-// workgroup.
-template <class P, int TILE>
-__device__ __forceinline__ void run_tile(int64_t base, int64_t end, const double* __restrict__ K,
-                                         const double* __restrict__ q, const DcsConsts& c,
-                                         const Outs<typename P::out_t>& o, TileLists<TILE, P::has_redo>& s,
-                                         const fm::Tab& t) {
+// Phase 1 of run_tile: each lane takes TILE / WG pairs with coalesced loads,
+// evaluates the closed forms and the exits of the two integrals, and appends
+// the pairs the integrals need to the tile's lists. ALL: every output is
+// requested (no per-output branches).
+template <class P, int TILE, bool ALL>
+__device__ __forceinline__ void phase1(int64_t base, int64_t end, const double* __restrict__ K,
+                                       const double* __restrict__ q, const DcsConsts& c,
+                                       const Outs<typename P::out_t>& o, TileLists<TILE, P::has_redo>& s,
+                                       const fm::Tab& t) {
     using out_t = typename P::out_t;
     constexpr int ITEMS = TILE / WG;
     const int tid = threadIdx.x;
-    if (tid == 0) {
-        s.n_photo = 0;
-        s.n_pair = 0;
-        s.n_redo = 0;
-    }
-    __syncthreads();
-
-    // phase 1: closed forms, exits, compaction
-    const bool want_pair = o.p != nullptr;
-    const bool want_photo = o.ph != nullptr;
+    const bool want_pair = ALL || o.p != nullptr;
+    const bool want_photo = ALL || o.ph != nullptr;
     int64_t gi = base + tid;
     double k_next = 0., q_next = 0.;
     if (gi < end) {
@@ -128,20 +121,65 @@ __device__ __forceinline__ void run_tile(int64_t base, int64_t end, const double
         // (mixed form) a pair outside the FP32 range goes whole to phase 3
         if constexpr (P::has_redo) full_redo = valid && !P::in_range(k, qq, c);
         if (valid && !full_redo) {
-            if (o.b) store_stream(&o.b[gi], P::brems(k, qq, c, t));
-            if (o.i) store_stream(&o.i[gi], P::ion(k, qq, c, t));
-            if (want_pair) {
+            if constexpr (ALL) {
+                const out_t b = P::brems(k, qq, c, t);
+                const out_t i = P::ion(k, qq, c, t);
                 pa = P::pair_active(k, qq, c);
-                if (!pa) store_stream(&o.p[gi], out_t(0));
-            }
-            if (want_photo) {
                 pha = P::photo_active(k, qq, c);
+                store_stream(&o.b[gi], b);
+                store_stream(&o.i[gi], i);
+                if (!pa) store_stream(&o.p[gi], out_t(0));
                 if (!pha) store_stream(&o.ph[gi], out_t(0));
+            } else {
+                if (o.b) store_stream(&o.b[gi], P::brems(k, qq, c, t));
+                if (o.i) store_stream(&o.i[gi], P::ion(k, qq, c, t));
+                if (want_pair) {
+                    pa = P::pair_active(k, qq, c);
+                    if (!pa) store_stream(&o.p[gi], out_t(0));
+                }
+                if (want_photo) {
+                    pha = P::photo_active(k, qq, c);
+                    if (!pha) store_stream(&o.ph[gi], out_t(0));
+                }
             }
         }
         if (want_photo) wave_append(pha, &s.n_photo, s.photo, uint32_t(local));
         if (want_pair) wave_append(pa, &s.n_pair, s.pair, uint32_t(local));
         if constexpr (P::has_redo) wave_append(full_redo, &s.n_redo, s.redo, uint32_t(local) | REDO_ALL);
+    }
+}
+
+// Evaluates pairs [base, end) (end - base <= TILE) of one element with the
+// log/exp tables `t` in LDS. Must be called by all WG threads of the
+// workgroup.
+template <class P, int TILE>
+__device__ __forceinline__ void run_tile(int64_t base, int64_t end, const double* __restrict__ K,
+                                         const double* __restrict__ q, const DcsConsts& c,
+                                         const Outs<typename P::out_t>& o, TileLists<TILE, P::has_redo>& s,
+                                         const fm::Tab& t) {
+    using out_t = typename P::out_t;
+    const int tid = threadIdx.x;
+    if (tid == 0) {
+        s.n_photo = 0;
+        s.n_pair = 0;
+        s.n_redo = 0;
+    }
+    __syncthreads();
+
+    // phase 1: closed forms, exits, compaction. In the FP64 form with all
+    // four outputs requested (the usual call) the four evaluations of a pair
+    // sit in one block, so the compiler shares what they have in common (the
+    // fast-path test, E = K + m and its reciprocal, 1 / q): 1.4% on the
+    // kernel. The mixed form keeps one version (a second raised its register
+    // spills and cost it 0.6%).
+    if constexpr (!P::has_redo) {
+        if (o.b && o.i && o.p && o.ph) {
+            phase1<P, TILE, true>(base, end, K, q, c, o, s, t);
+        } else {
+            phase1<P, TILE, false>(base, end, K, q, c, o, s, t);
+        }
+    } else {
+        phase1<P, TILE, false>(base, end, K, q, c, o, s, t);
     }
     __syncthreads();
 
