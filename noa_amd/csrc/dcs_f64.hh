@@ -188,8 +188,16 @@ struct F64N {
         return pair_active_impl<true>(K, q, c);
     }
 
+    // The same, also returning the lower end of the integral, argmin (the
+    // value pair_impl computes for it: both quotient forms round as the
+    // division for an active pair's normal operands), for pair_at.
+    static NOA_DEV_INLINE bool pair_active(double K, double q, const DcsConsts& c, double& argmin) {
+        if (ordinary(K, q, c)) return pair_active_impl<false>(K, q, c, &argmin);
+        return pair_active_impl<true>(K, q, c, &argmin);
+    }
+
     template <bool Safe>
-    static NOA_DEV_INLINE bool pair_active_impl(double K, double q, const DcsConsts& c) {
+    static NOA_DEV_INLINE bool pair_active_impl(double K, double q, const DcsConsts& c, double* argmin_out = nullptr) {
 #pragma clang fp contract(off)
         if (q <= 4. * ELECTRON_MASS) return false;
         if (q >= K + c.p_cut) return false;
@@ -206,6 +214,7 @@ struct F64N {
             if (fabs(Da) < 0x1p-900) Da = 0x1p-900;
         }
         const double argmin = dv<Safe>(Na, Da, Safe ? 0. : rcp<Safe>(Da));
+        if (argmin_out) *argmin_out = argmin;
         return !((argmin >= 1.) || (argmin <= 0.));
     }
 
@@ -231,6 +240,13 @@ struct F64N {
         return pair_impl<true>(K, q, c, t);  // also infinite and NaN K
     }
 
+    // The same from the argmin pair_active returned (no second evaluation of
+    // the exit's quotients and square root); the same result.
+    static NOA_DEV double pair_at(double K, double q, double argmin, const DcsConsts& c, const fm::Tab& t) {
+        if (c.fast_ok && K + c.mass <= FAST_E_MAX) return pair_impl<false, true, true>(K, q, c, t, argmin);
+        return pair_impl<true, false, true>(K, q, c, t, argmin);
+    }
+
     // The same with the set-up quotients as IEEE divisions (Rn = false): the
     // FP64 recomputation inside the mixed-precision kernel, where the
     // div_rn path's extra constants raise that kernel's scalar register
@@ -240,25 +256,27 @@ struct F64N {
         return pair_impl<true>(K, q, c, t);
     }
 
-    template <bool Safe, bool Rn = !Safe>
-    static NOA_DEV double pair_impl(double K, double q, const DcsConsts& c, const fm::Tab& t) {
+    template <bool Safe, bool Rn = !Safe, bool HaveArg = false>
+    static NOA_DEV double pair_impl(double K, double q, const DcsConsts& c, const fm::Tab& t, double argmin_in = 0.) {
 #pragma clang fp contract(off)
         const double me = ELECTRON_MASS;
         // the set-up quotients as in pair_active (a pair that passed it has
         // 4 m_e < q < K + p_cut, p_cut < 0, so K, q > 2e-3 GeV, E - q > 0.76 m
         // and argmin in (0, 1): every divisor below is positive normal)
         const double E0 = K + c.mass;
-        const double rq = !Rn ? 0. : rcp<!Rn>(q);
         const double nu = dv<!Rn>(q, E0, !Rn ? 0. : rcp<!Rn>(E0));
         const double omn = 1. - nu;
         const double beta = dv<!Rn>(0.5 * nu * nu, omn, !Rn ? 0. : rcp<!Rn>(omn));
         const double xi_factor = c.p_hr2 * beta;
         const double gamma = 1. + dv<!Rn>(K, c.mass, c.r_mass);
-        const double x0 = dv<!Rn>(4. * me, q, rq);
-        const double gg = gamma * (gamma - dv<!Rn>(q, c.mass, c.r_mass));
-        const double x1 = dv<!Rn>(6., gg, !Rn ? 0. : rcp<!Rn>(gg));
-        const double Da = 1. + (1. - x1) * sqrt(1. - x0);
-        const double argmin = dv<!Rn>(x0 + 2. * (1. - x0) * x1, Da, !Rn ? 0. : rcp<!Rn>(Da));
+        double argmin = argmin_in;
+        if constexpr (!HaveArg) {
+            const double x0 = dv<!Rn>(4. * me, q, !Rn ? 0. : rcp<!Rn>(q));
+            const double gg = gamma * (gamma - dv<!Rn>(q, c.mass, c.r_mass));
+            const double x1 = dv<!Rn>(6., gg, !Rn ? 0. : rcp<!Rn>(gg));
+            const double Da = 1. + (1. - x1) * sqrt(1. - x0);
+            argmin = dv<!Rn>(x0 + 2. * (1. - x0) * x1, Da, !Rn ? 0. : rcp<!Rn>(Da));
+        }
         const double tmin = nlog<Safe>(argmin, t);  // argmin in (0, 1), >= x0 / 2 > 1e-33
 
         // per-pair coefficients of the node polynomials in rho^2 (explicit

@@ -42,7 +42,10 @@ void eval_range(int64_t lo, int64_t hi, const double* K, const double* q, const 
     for (int64_t i = lo; i < hi; i++) {
         const double k = K[i], qq = q[i];
         if (o.brems) o.brems[i] = F64::brems(k, qq, c, t);
-        if (o.pair) o.pair[i] = F64::pair_active(k, qq, c) ? F64::pair(k, qq, c, t) : 0.;
+        if (o.pair) {
+            double am;
+            o.pair[i] = F64::pair_active(k, qq, c, am) ? F64::pair_at(k, qq, am, c, t) : 0.;
+        }
         if (o.photo) o.photo[i] = F64::photo_active(k, qq, c) ? F64::photo(k, qq, c, t) : 0.;
         if (o.ion) o.ion[i] = F64::ion(k, qq, c, t);
     }

@@ -41,10 +41,13 @@ struct Outs {
 // The tile's lists in LDS. A pair enters the redo list at most twice (once
 // for all four processes from phase 1, or once per failed integral from
 // phase 2), so a form with a redo list holds 2 TILE entries; the FP64 form
-// holds none.
+// holds none. The FP64 form keeps, beside each pair-production entry, the
+// lower end of its integral (argmin), which the exit test of phase 1 has
+// already computed.
 template <int TILE, bool REDO>
 struct TileLists {
     static_assert(TILE <= 16384, "14-bit tile offsets (bits 14 and 15 tag the process in the redo list)");
+    double pair_arg[REDO ? 1 : TILE];
     uint16_t photo[TILE];
     uint16_t pair[TILE];
     uint16_t redo[REDO ? 2 * TILE : 1];
@@ -59,16 +62,18 @@ constexpr uint32_t REDO_OFFSET = 0x3FFFu;
 
 // Appends `value` for every active lane with pred set; one LDS atomic per
 // wave. Lanes outside the current EXEC mask take no part, so it may be called
-// from divergent code as long as every lane with pred set is active.
-__device__ __forceinline__ void wave_append(bool pred, uint32_t* counter, uint16_t* list, uint32_t value) {
+// from divergent code as long as every lane with pred set is active. Returns
+// the entry's position in the list (meaningful where pred is set).
+__device__ __forceinline__ uint32_t wave_append(bool pred, uint32_t* counter, uint16_t* list, uint32_t value) {
     const unsigned long long m = __ballot(pred);
-    if (m == 0ull) return;
+    if (m == 0ull) return 0u;
     const uint32_t rank = __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
     uint32_t base = 0;
     if (pred && rank == 0) base = atomicAdd(counter, uint32_t(__popcll(m)));
     const int leader = __ffsll((long long)m) - 1;
     base = uint32_t(__builtin_amdgcn_readlane(int(base), leader));
     if (pred) list[base + rank] = uint16_t(value);
+    return base + rank;
 }
 
 // Output stores are non-temporal: the results are not read again by the
@@ -118,13 +123,14 @@ __device__ __forceinline__ void phase1(int64_t base, int64_t end, const double* 
             q_next = q[gi + WG];
         }
         bool pa = false, pha = false, full_redo = false;
+        double am = 0.;  // (FP64 form) argmin of an active pair-production integral
         // (mixed form) a pair outside the FP32 range goes whole to phase 3
         if constexpr (P::has_redo) full_redo = valid && !P::in_range(k, qq, c);
         if (valid && !full_redo) {
             if constexpr (ALL) {
                 const out_t b = P::brems(k, qq, c, t);
                 const out_t i = P::ion(k, qq, c, t);
-                pa = P::pair_active(k, qq, c);
+                pa = P::pair_active(k, qq, c, am);
                 pha = P::photo_active(k, qq, c);
                 store_stream(&o.b[gi], b);
                 store_stream(&o.i[gi], i);
@@ -134,7 +140,10 @@ __device__ __forceinline__ void phase1(int64_t base, int64_t end, const double* 
                 if (o.b) store_stream(&o.b[gi], P::brems(k, qq, c, t));
                 if (o.i) store_stream(&o.i[gi], P::ion(k, qq, c, t));
                 if (want_pair) {
-                    pa = P::pair_active(k, qq, c);
+                    if constexpr (P::has_redo)
+                        pa = P::pair_active(k, qq, c);
+                    else
+                        pa = P::pair_active(k, qq, c, am);
                     if (!pa) store_stream(&o.p[gi], out_t(0));
                 }
                 if (want_photo) {
@@ -144,7 +153,11 @@ __device__ __forceinline__ void phase1(int64_t base, int64_t end, const double* 
             }
         }
         if (want_photo) wave_append(pha, &s.n_photo, s.photo, uint32_t(local));
-        if (want_pair) wave_append(pa, &s.n_pair, s.pair, uint32_t(local));
+        if (want_pair) {
+            const uint32_t at = wave_append(pa, &s.n_pair, s.pair, uint32_t(local));
+            if constexpr (!P::has_redo)
+                if (pa) s.pair_arg[at] = am;
+        }
         if constexpr (P::has_redo) wave_append(full_redo, &s.n_redo, s.redo, uint32_t(local) | REDO_ALL);
     }
 }
@@ -198,7 +211,10 @@ __device__ __forceinline__ void run_tile(int64_t base, int64_t end, const double
         } else {
             tag = s.pair[v - nph];
             const int64_t g = base + tag;
-            store_stream(&o.p[g], P::pair(K[g], q[g], c, redo, t));
+            if constexpr (P::has_redo)
+                store_stream(&o.p[g], P::pair(K[g], q[g], c, redo, t));
+            else
+                store_stream(&o.p[g], P::pair_at(K[g], q[g], s.pair_arg[v - nph], c, t));
         }
         if (P::has_redo) wave_append(redo, &s.n_redo, s.redo, tag);
     }

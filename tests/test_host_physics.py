@@ -36,6 +36,7 @@ def host_eval(tmp_path_factory):
         lib.dcs_host_eval(K.ctypes.data_as(D), q.ctypes.data_as(D), K.size, el.A, el.I, el.Z, mass,
                           out.ctypes.data_as(D))
         return out.T
+    run.lib = lib
     return run
 
 
@@ -94,3 +95,32 @@ def test_special_inputs(host_eval):
             fin = np.isfinite(g) & np.isfinite(e) & (e != 0)
             same[fin] |= np.abs(g[fin] - e[fin]) <= 1e-10 * np.abs(e[fin])
             assert same.all(), (el.Z, p, K[~same][:4], q[~same][:4], g[~same][:4], e[~same][:4])
+
+
+def test_pair_from_the_exit_argmin_is_bitwise_the_same(host_eval):
+    """The kernels take pair production's lower integration bound from the
+    exit test (pair_active) instead of computing it again: the results are
+    bit for bit those of the self-contained evaluation, over ordinary,
+    extreme-energy (full-range path) and near-endpoint pairs, four elements
+    and four masses."""
+    lib = host_eval.lib
+    D = ctypes.POINTER(ctypes.c_double)
+    lib.dcs_host_pair_routes.argtypes = [D, D, ctypes.c_long, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                         ctypes.c_double, D]
+    rng = np.random.default_rng(17)
+    n = 1 << 16
+    K = 10 ** rng.uniform(-3, 33, n)
+    q = K * 10 ** rng.uniform(-8, 0, n)
+    q[::7] = K[::7] * (1 - 10 ** rng.uniform(-15, -1, q[::7].size))
+    els = [STANDARD_ROCK, AtomicElement(A=1.00794, I=19.2e-9, Z=1), AtomicElement(A=238.02891, I=890e-9, Z=92),
+           AtomicElement(A=0.9, I=19.2e-9, Z=1)]
+    active = 0
+    for el in els:
+        for mass in (MUON_MASS, TAU_MASS, 2e6, 5e-7):
+            out = np.empty(2 * n)
+            lib.dcs_host_pair_routes(K.ctypes.data_as(D), q.ctypes.data_as(D), n, el.A, el.I, el.Z, mass,
+                                     out.ctypes.data_as(D))
+            a, b = out[0::2], out[1::2]
+            assert np.array_equal(a.view(np.uint64), b.view(np.uint64)), (el, mass)
+            active += int((a != 0).sum())
+    assert active > n
