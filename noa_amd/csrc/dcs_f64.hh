@@ -155,7 +155,15 @@ struct F64N {
         return ion_impl<true>(K, q, c, t);
     }
 
-    template <bool Safe>
+    // The same for the mixed-precision kernel's FP64 recomputation, with the
+    // full log for 4 E (E - q) / m^2 (its register allocation is the tighter
+    // one; the value is the same)
+    static NOA_DEV_INLINE double ion_redo(double K, double q, const DcsConsts& c, const fm::Tab& t) {
+        if (ordinary(K, q, c)) return ion_impl<false, false>(K, q, c, t);
+        return ion_impl<true>(K, q, c, t);
+    }
+
+    template <bool Safe, bool Lean = !Safe>
     static NOA_DEV_INLINE double ion_impl(double K, double q, const DcsConsts& c, const fm::Tab& t) {
 #pragma clang fp contract(off)
         const double me = ELECTRON_MASS;
@@ -175,9 +183,12 @@ struct F64N {
         if (K >= c.i_Kthr) {
             // 1 + 2q/m_e >= 1: log_pos on the fast path. E - q can round to 0
             // (E = K + m == K for K >> m, q = Wmax = K), where the reference's
-            // log gives -inf: the full log there
+            // log gives -inf; otherwise E - q >= ulp(E) keeps 4 E (E - q) / m^2
+            // positive normal, so the fast path needs no other special case
             const double L1 = nlog<Safe>(1. + dv<Safe>(2. * q, me, R_ME), t);
-            Delta = 1.16141E-03 * L1 * (fm::log(dv<Safe>(4. * E * (E - q), c.i_m2, c.i_r_m2), t) - L1);
+            const double a4 = dv<Safe>(4. * E * (E - q), c.i_m2, c.i_r_m2);
+            const double L4 = Lean ? ((a4 > 0.) ? fm::log_pos(a4, t) : -INFINITY) : fm::log(a4, t);
+            Delta = 1.16141E-03 * L1 * (L4 - L1);
         }
         return cs * (1. + Delta);
     }

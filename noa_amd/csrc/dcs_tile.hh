@@ -230,7 +230,7 @@ __device__ __forceinline__ void run_tile(int64_t base, int64_t end, const double
             if (tag & REDO_ALL) {
                 const double k = K[g], qq = q[g];
                 if (o.b) o.b[g] = out_t(F64::brems(k, qq, c, t));
-                if (o.i) o.i[g] = out_t(F64::ion(k, qq, c, t));
+                if (o.i) o.i[g] = out_t(F64::ion_redo(k, qq, c, t));
                 if (o.p) o.p[g] = out_t(F64::pair_active(k, qq, c) ? F64::pair_ieee(k, qq, c, t) : 0.);
                 if (o.ph) o.ph[g] = out_t(F64::photo_active(k, qq, c) ? F64::photo_ieee(k, qq, c, t) : 0.);
             } else if (tag & REDO_PHOTO) {
