@@ -8,7 +8,8 @@
 //      closed-form processes (bremsstrahlung, ionisation), runs the cheap
 //      kinematic exits of the two integral processes, writes 0 where they
 //      exit, and appends the surviving pair to a per-process list in LDS
-//      (wave ballot + one LDS atomic per wave);
+//      (wave ballot + one LDS atomic per wave; the FP64 form keeps pair
+//      production's argmin, computed by its exit test, beside the entry);
 //   2. all lanes drain the two lists densely, heaviest process first, so
 //      every wave runs the integrals with (almost) all lanes active;
 //   3. (mixed precision only) pairs outside the FP32 form's range were
