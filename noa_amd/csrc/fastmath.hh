@@ -262,5 +262,28 @@ NOA_HD inline double exp_fin(double x, const Tab& t) {
     return std::ldexp(y, m);
 }
 
+// exp_fin with a polynomial one degree shorter: e^r - 1 = r (1 + r Q(r)),
+// Q the degree-2 Chebyshev economization of (e^r - 1 - r) / r^2 over the
+// reduced range (truncation < 1.4 ulp; measured within 3 ulp in all against
+// 0.6 for exp_fin), one FMA less. Same argument range as exp_fin. For factors whose
+// relative error passes through to the result unamplified.
+NOA_HD inline double exp_fin4(double x, const Tab& t) {
+#pragma clang fp contract(off)
+    const double shifted = std::fma(x, INV_LN2_N, 0x1.8p52);
+    const int ki = int(uint32_t(as_bits(shifted)));
+    const double kd = shifted - 0x1.8p52;
+    double r = fnma_kb(kd, LN2_N_HI, x);
+    r = fnma_kb(kd, LN2_N_LO, r);
+    const int j = ki & (EXP_N - 1);
+    const int m = ki >> 7;
+    double p = fma_kc(r, EXP4_P2, EXP4_P1);
+    p = fma_kc(p, r, EXP4_P0);
+    p = std::fma(p, r, 1.0);
+    p = p * r;
+    const double th = t.ehi[j];
+    const double y = th + std::fma(th, p, t.elo[j]);
+    return std::ldexp(y, m);
+}
+
 }  // namespace fm
 }  // namespace noa_hip

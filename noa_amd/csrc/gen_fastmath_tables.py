@@ -21,6 +21,8 @@ log1p(r) = r + r^2 P(r): P is the degree-4 Chebyshev economization of the
   to powers of r), coefficients rounded once; the truncation error is below
   0.07 * 2^-53 relative to log1p(r) (checked here against 60-digit ln).
 exp: T[j] = 2^(j/128), j < 128, as a double-double (hi, lo).
+shorter exp: the degree-2 Chebyshev economization Q of (e^r - 1 - r) / r^2
+  over the same r range (truncation below 1.5 ulp, checked here).
 """
 from __future__ import annotations
 
@@ -173,6 +175,31 @@ def h_poly(deg: int = 6, hi: Fraction = Fraction(1, 4), terms: int = 60):
     return coef
 
 
+def exp4_poly(deg: int = 2, terms: int = 30):
+    """Coefficients (constant first) of Q(r) ~ (e^r - 1 - r) / r^2 on [-h, h],
+    h = ln2 / (2 N_EXP) with a margin for the rounding of k: the degree-2
+    Chebyshev economization of sum_n r^n / (n + 2)! (the series of Q), for the
+    shorter exp (exp_fin4: e^r - 1 = r (1 + r Q(r))). Truncation error checked
+    below 1.5 ulp of e^r (2^-52 relative units)."""
+    from math import factorial
+    ln2 = Fraction(Decimal(2).ln())
+    h = ln2 / (2 * N_EXP) * (1 + Fraction(1, 1 << 20))
+    ps = [Fraction(1, factorial(n + 2)) * h ** n for n in range(terms + 1)]  # the series in s, r = h s
+    p_s = _chebyshev_to_power(_to_chebyshev(ps)[:deg + 1])
+    coef = [float(v * (1 / h) ** k) for k, v in enumerate(p_s)]
+    worst = Decimal(0)
+    for i in range(-1000, 1001):
+        r = h * Fraction(i, 1000)
+        q = Fraction(0)
+        for v in reversed(coef):
+            q = q * r + Fraction(v)
+        approx = 1 + r + r * r * q
+        exact = (Decimal(r.numerator) / Decimal(r.denominator)).exp()
+        worst = max(worst, abs((Decimal(approx.numerator) / Decimal(approx.denominator) - exact) / exact))
+    assert worst < Decimal(1.5) * Decimal(2) ** -52, worst
+    return coef
+
+
 def exp_tables():
     th, tl = [], []
     for j in range(N_EXP):
@@ -226,6 +253,9 @@ def main() -> None:
     out.append("// log1p(r) = r + r^2 (LOG_P0 + r (LOG_P1 + ...)), r in [-2^-9, 2^-8]")
     for k, v in enumerate(poly):
         out.append(f"constexpr double LOG_P{k} = {float.hex(v)};")
+    out.append("// shorter exp: e^r - 1 = r (1 + r (EXP4_P0 + r (EXP4_P1 + r EXP4_P2))), |r| <= ln2 / 256")
+    for k, v in enumerate(exp4_poly()):
+        out.append(f"constexpr double EXP4_P{k} = {float.hex(v)};")
     out.append("// FP32: (log1p(u) - u) / u^2 = H_P0 + u (H_P1 + ...), u in [0, 1/4]")
     for k, v in enumerate(h_poly()):
         out.append(f"constexpr float H_P{k} = {float.hex(v)}f;")

@@ -37,6 +37,11 @@ extern "C" void fm_exp_fin_batch(const double* x, double* y, long n) {
     for (long i = 0; i < n; i++) y[i] = noa_hip::fm::exp_fin(x[i], t);
 }
 
+extern "C" void fm_exp_fin4_batch(const double* x, double* y, long n) {
+    const auto t = noa_hip::fm::tab_from_block(block().v);
+    for (long i = 0; i < n; i++) y[i] = noa_hip::fm::exp_fin4(x[i], t);
+}
+
 // a / b through div_rn with the correctly rounded reciprocal
 extern "C" void fm_div_rn_batch(const double* a, const double* b, double* y, long n) {
     for (long i = 0; i < n; i++) y[i] = noa_hip::fm::div_rn(a[i], b[i], 1.0 / b[i]);

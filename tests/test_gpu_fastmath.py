@@ -16,7 +16,7 @@ pytestmark = pytest.mark.gpu
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 DEV_SO = os.path.join(HERE, "native", "libfm_dev.so")
-FN = {"log": 0, "log_pos": 1, "exp": 2, "exp_fin": 3, "recip_pos": 4, "recip_nz": 5}
+FN = {"log": 0, "log_pos": 1, "exp": 2, "exp_fin": 3, "recip_pos": 4, "recip_nz": 5, "exp_fin4": 6}
 
 
 @pytest.fixture(scope="module")
@@ -58,7 +58,7 @@ def host(tmp_path_factory):
     lib = ctypes.CDLL(so)
     D = ctypes.POINTER(ctypes.c_double)
     fns = {"log": lib.fm_log_batch, "log_pos": lib.fm_log_pos_batch, "exp": lib.fm_exp_batch,
-           "exp_fin": lib.fm_exp_fin_batch}
+           "exp_fin": lib.fm_exp_fin_batch, "exp_fin4": lib.fm_exp_fin4_batch}
     for f in fns.values():
         f.argtypes = [D, D, ctypes.c_long]
 
@@ -130,7 +130,7 @@ def test_log_exp_match_host_build(dev, host):
         np.testing.assert_array_equal(dev(fn, xl), host(fn, xl), err_msg=fn)
     xe = np.concatenate([rng.uniform(-745, 709.7, 1 << 20), rng.uniform(-1, 1, 1 << 18),
                          rng.uniform(-1e-8, 1e-8, 1 << 16)])
-    for fn in ("exp", "exp_fin"):
+    for fn in ("exp", "exp_fin", "exp_fin4"):
         np.testing.assert_array_equal(dev(fn, xe), host(fn, xe), err_msg=fn)
     sl = np.array([0.0, -0.0, -1.0, np.inf, -np.inf, np.nan, 5e-324, 1e-310, 2.2250738585072014e-308])
     se = np.array([-np.inf, np.inf, np.nan, -750.0, 750.0, 1e300, -1e300, 709.8, -745.2])
