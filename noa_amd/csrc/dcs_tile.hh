@@ -8,8 +8,9 @@
 //      closed-form processes (bremsstrahlung, ionisation), runs the cheap
 //      kinematic exits of the two integral processes, writes 0 where they
 //      exit, and appends the surviving pair to a per-process list in LDS
-//      (wave ballot + one LDS atomic per wave; the FP64 form keeps pair
-//      production's argmin, computed by its exit test, beside the entry);
+//      (wave ballot + one LDS atomic per wave; the FP64 form assembles the
+//      pair-production plane in LDS, starting from the argmin of its exit
+//      test, and writes it out coalesced after phase 2);
 //   2. all lanes drain the two lists densely, heaviest process first, so
 //      every wave runs the integrals with (almost) all lanes active;
 //   3. (mixed precision only) pairs outside the FP32 form's range were
@@ -42,13 +43,15 @@ struct Outs {
 // The tile's lists in LDS. A pair enters the redo list at most twice (once
 // for all four processes from phase 1, or once per failed integral from
 // phase 2), so a form with a redo list holds 2 TILE entries; the FP64 form
-// holds none. The FP64 form keeps, beside each pair-production entry, the
-// lower end of its integral (argmin), which the exit test of phase 1 has
-// already computed.
+// holds none. The FP64 form assembles the tile's pair-production plane in
+// LDS (pair_out, by tile offset): the lower end of the integral (argmin),
+// which the exit test of phase 1 has already computed, until phase 2
+// replaces it with the result, and the exit's 0 where the integral does not
+// run; the plane is then written out with coalesced stores.
 template <int TILE, bool REDO>
 struct TileLists {
     static_assert(TILE <= 16384, "14-bit tile offsets (bits 14 and 15 tag the process in the redo list)");
-    double pair_arg[REDO ? 1 : TILE];
+    double pair_out[REDO ? 1 : TILE];
     uint16_t photo[TILE];
     uint16_t pair[TILE];
     uint16_t redo[REDO ? 2 * TILE : 1];
@@ -135,17 +138,17 @@ __device__ __forceinline__ void phase1(int64_t base, int64_t end, const double* 
                 pha = P::photo_active(k, qq, c);
                 store_stream(&o.b[gi], b);
                 store_stream(&o.i[gi], i);
-                if (!pa) store_stream(&o.p[gi], out_t(0));
                 if (!pha) store_stream(&o.ph[gi], out_t(0));
             } else {
                 if (o.b) store_stream(&o.b[gi], P::brems(k, qq, c, t));
                 if (o.i) store_stream(&o.i[gi], P::ion(k, qq, c, t));
                 if (want_pair) {
-                    if constexpr (P::has_redo)
+                    if constexpr (P::has_redo) {
                         pa = P::pair_active(k, qq, c);
-                    else
+                        if (!pa) store_stream(&o.p[gi], out_t(0));
+                    } else {
                         pa = P::pair_active(k, qq, c, am);
-                    if (!pa) store_stream(&o.p[gi], out_t(0));
+                    }
                 }
                 if (want_photo) {
                     pha = P::photo_active(k, qq, c);
@@ -155,9 +158,12 @@ __device__ __forceinline__ void phase1(int64_t base, int64_t end, const double* 
         }
         if (want_photo) wave_append(pha, &s.n_photo, s.photo, uint32_t(local));
         if (want_pair) {
-            const uint32_t at = wave_append(pa, &s.n_pair, s.pair, uint32_t(local));
+            wave_append(pa, &s.n_pair, s.pair, uint32_t(local));
+            // (FP64 form) the pair-production plane of the tile is assembled in
+            // LDS: argmin where the integral runs (replaced by its result in
+            // phase 2), the exit's 0 elsewhere; written out coalesced at the end
             if constexpr (!P::has_redo)
-                if (pa) s.pair_arg[at] = am;
+                if (valid) s.pair_out[local] = pa ? am : 0.;
         }
         if constexpr (P::has_redo) wave_append(full_redo, &s.n_redo, s.redo, uint32_t(local) | REDO_ALL);
     }
@@ -215,9 +221,17 @@ __device__ __forceinline__ void run_tile(int64_t base, int64_t end, const double
             if constexpr (P::has_redo)
                 store_stream(&o.p[g], P::pair(K[g], q[g], c, redo, t));
             else
-                store_stream(&o.p[g], P::pair_at(K[g], q[g], s.pair_arg[v - nph], c, t));
+                s.pair_out[tag] = P::pair_at(K[g], q[g], s.pair_out[tag], c, t);
         }
         if (P::has_redo) wave_append(redo, &s.n_redo, s.redo, tag);
+    }
+
+    // (FP64 form) the assembled pair-production plane, coalesced
+    if constexpr (!P::has_redo) {
+        if (o.p) {
+            __syncthreads();
+            for (int64_t gi = base + tid; gi < end; gi += WG) store_stream(&o.p[gi], s.pair_out[gi - base]);
+        }
     }
 
     // phase 3: FP64 recomputation of the pairs that failed their estimate
