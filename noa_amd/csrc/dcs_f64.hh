@@ -509,6 +509,12 @@ struct F64N {
             else
                 return nexp<Safe>(v, tb);
         };
+        const auto plog = [](double v, const fm::Tab& tb) {
+            if constexpr (Rn)
+                return fm::log_pos_lo(v, tb);
+            else
+                return nlog<Safe>(v, tb);
+        };
         double ds = 0.;
 #pragma unroll NU
         for (int j = 0; j < 9; j++) {
@@ -520,7 +526,7 @@ struct F64N {
             // dcs_photonuclear_d2 (dcs.hh:335-355)
             const double x = Q2 * inv_2Mq;
             // dcs_photonuclear_f2_allm (dcs.hh:261-307)
-            const double tl = nlog<Safe>(nlog<Safe>((Q2 + ALLM_Q02) * inv_l2, t) * c.ph_inv_L0, t);
+            const double tl = plog(plog((Q2 + ALLM_Q02) * inv_l2, t) * c.ph_inv_L0, t);
             const double xP = Safe ? (Q2 + 49.457) * inv_dP : fma(Q2, inv_dP, cxP);
             const double xR = Safe ? (Q2 + 0.15052) * inv_dR : fma(Q2, inv_dR, cxR);
             // t -> 0 (Q^2 -> 0) gives log t = -inf and t^c = 0; clamped at -745
@@ -529,7 +535,7 @@ struct F64N {
             // negative and NaN t included): log t >= -708.4 there, where every
             // t^c rounds to 0 against those constants too, so the results are
             // the same and the log needs no special-argument path
-            const double lnt = Rn ? fm::log_pos(fmax(tl, 0x1p-1022), t) : fmax(fm::log(tl, t), -745.0);
+            const double lnt = Rn ? fm::log_pos_lo(fmax(tl, 0x1p-1022), t) : fmax(fm::log(tl, t), -745.0);
             const double gP = 1.0 + pexp(2.1979 * lnt, t), gA = 1.0 + pexp(1.1709 * lnt, t);
             const double bP = 0.36292 + 1.8917 * pexp(1.8439 * lnt, t);
             const double cR = 0.80107 + 0.97307 * pexp(3.4942 * lnt, t);
@@ -575,8 +581,8 @@ struct F64N {
             }
             const double cP = 0.28067 + (0.28067 - 0.22291) * (Safe ? gA * rG - 1.0 : fma(gA, rG, -1.0));
             const double aP = -0.0808 + (-0.0808 - -0.44812) * (Safe ? gP * rG - 1.0 : fma(gP, rG, -1.0));
-            const double l1x = nlog<Safe>(1. - x, t);
-            const double lxP = nlog<Safe>(xP, t), lxR = nlog<Safe>(xR, t);
+            const double l1x = plog(1. - x, t);
+            const double lxP = plog(xP, t), lxR = plog(xR, t);
             const double F2P = cP * pexp(Safe ? aP * lxP + bP * l1x : fma(aP, lxP, bP * l1x), t);
             const double F2R = cR * pexp(Safe ? aR * lxR + bR * l1x : fma(aR, lxR, bR * l1x), t);
             const double F2p = Q2 * inv_Qm * (F2P + F2R);

@@ -262,6 +262,28 @@ NOA_HD inline double exp_fin(double x, const Tab& t) {
     return std::ldexp(y, m);
 }
 
+// log_pos without the r^4 term of its polynomial (the same table and
+// constants otherwise): absolute error < 6.5e-16 (the dropped term is at
+// most |LOG_P4| r^6 <= 6e-16 over |r| <= 2^-8), one FMA less. For
+// logarithms whose absolute error is what counts (added into exponents).
+NOA_HD inline double log_pos_lo(double x, const Tab& t) {
+#pragma clang fp contract(off)
+    const uint64_t ix = as_bits(x);
+    const uint32_t hx = uint32_t(ix >> 32);
+    const uint32_t ht = hx - uint32_t(LOG_OFF >> 32);
+    const int i = int((ht >> (LOG_SHIFT - 32)) & uint32_t(LOG_N - 1));
+    const int k = int32_t(ht) >> 20;
+    const double z = from_bits((uint64_t(hx - (ht & 0xfff00000u)) << 32) | (ix & 0xffffffffull));
+    const double kd = double(k);
+    const double r = std::fma(z, t.invc[i], -1.0);
+    const double w = fma_kb(kd, LN2_HI, t.lhi[i]);
+    double q = fma_kc(r, LOG_P3, LOG_P2);
+    q = fma_kc(q, r, LOG_P1);
+    q = fma_kc(q, r, LOG_P0);
+    const double lo = fma_kb(kd, LN2_LO, t.llo[i]);
+    return w + (r + std::fma(r * r, q, lo));
+}
+
 // exp_fin with a polynomial one degree shorter: e^r - 1 = r (1 + r Q(r)),
 // Q the degree-2 Chebyshev economization of (e^r - 1 - r) / r^2 over the
 // reduced range (truncation < 1.4 ulp; measured within 3 ulp in all against

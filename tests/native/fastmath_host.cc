@@ -37,6 +37,11 @@ extern "C" void fm_exp_fin_batch(const double* x, double* y, long n) {
     for (long i = 0; i < n; i++) y[i] = noa_hip::fm::exp_fin(x[i], t);
 }
 
+extern "C" void fm_log_pos_lo_batch(const double* x, double* y, long n) {
+    const auto t = noa_hip::fm::tab_from_block(block().v);
+    for (long i = 0; i < n; i++) y[i] = noa_hip::fm::log_pos_lo(x[i], t);
+}
+
 extern "C" void fm_exp_fin4_batch(const double* x, double* y, long n) {
     const auto t = noa_hip::fm::tab_from_block(block().v);
     for (long i = 0; i < n; i++) y[i] = noa_hip::fm::exp_fin4(x[i], t);

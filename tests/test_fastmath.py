@@ -37,8 +37,10 @@ def fm(tmp_path_factory):
         return y
     lib.fm_exp_fin_batch.argtypes = [D, D, ctypes.c_long]
     lib.fm_exp_fin4_batch.argtypes = [D, D, ctypes.c_long]
+    lib.fm_log_pos_lo_batch.argtypes = [D, D, ctypes.c_long]
     return {"log": lambda x: call(lib.fm_log_batch, x), "exp": lambda x: call(lib.fm_exp_batch, x), "div": div,
-            "exp_fin": lambda x: call(lib.fm_exp_fin_batch, x), "exp_fin4": lambda x: call(lib.fm_exp_fin4_batch, x)}
+            "exp_fin": lambda x: call(lib.fm_exp_fin_batch, x), "exp_fin4": lambda x: call(lib.fm_exp_fin4_batch, x),
+            "log_pos_lo": lambda x: call(lib.fm_log_pos_lo_batch, x)}
 
 
 def ulp_err(got: np.ndarray, exact: list) -> np.ndarray:
@@ -160,3 +162,19 @@ def test_exp_fin4_ulp_vs_exact(fm):
     x = np.concatenate([rng.uniform(-700, 700, 1500), rng.uniform(-1, 1, 1500), rng.uniform(-1e-6, 1e-6, 200)])
     err = ulp_err(fm["exp_fin4"](x), [Decimal(float(v)).exp() for v in x])
     assert err.max() < 3.5 and err.mean() < 0.6, (err.max(), err.mean())
+
+
+def test_log_pos_lo_absolute_error(fm):
+    """log_pos without the r^4 term (for logarithms added into exponents):
+    absolute error within 6.5e-16 where |log x| < 1 (near 1 included) and
+    within 1.5 ulp where |log x| >= 1."""
+    getcontext().prec = 60
+    rng = np.random.default_rng(15)
+    x = np.concatenate([10 ** rng.uniform(-300, 300, 1500), rng.uniform(0.5, 2.0, 1500),
+                        1 + rng.uniform(-1e-3, 1e-3, 500), 1 + rng.uniform(-2 ** -8, 2 ** -8, 500)])
+    got = fm["log_pos_lo"](x)
+    exact = [Decimal(float(v)).ln() for v in x]
+    ab = np.array([float(abs(Decimal(float(g)) - e)) for g, e in zip(got, exact)])
+    big = np.abs(np.log(x)) >= 1
+    assert ab[~big].max() < 6.5e-16, ab[~big].max()
+    assert ulp_err(got[big], [e for e, b in zip(exact, big) if b]).max() < 1.5

@@ -16,7 +16,7 @@ pytestmark = pytest.mark.gpu
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 DEV_SO = os.path.join(HERE, "native", "libfm_dev.so")
-FN = {"log": 0, "log_pos": 1, "exp": 2, "exp_fin": 3, "recip_pos": 4, "recip_nz": 5, "exp_fin4": 6}
+FN = {"log": 0, "log_pos": 1, "exp": 2, "exp_fin": 3, "recip_pos": 4, "recip_nz": 5, "exp_fin4": 6, "log_pos_lo": 7}
 
 
 @pytest.fixture(scope="module")
@@ -58,7 +58,8 @@ def host(tmp_path_factory):
     lib = ctypes.CDLL(so)
     D = ctypes.POINTER(ctypes.c_double)
     fns = {"log": lib.fm_log_batch, "log_pos": lib.fm_log_pos_batch, "exp": lib.fm_exp_batch,
-           "exp_fin": lib.fm_exp_fin_batch, "exp_fin4": lib.fm_exp_fin4_batch}
+           "exp_fin": lib.fm_exp_fin_batch, "exp_fin4": lib.fm_exp_fin4_batch,
+           "log_pos_lo": lib.fm_log_pos_lo_batch}
     for f in fns.values():
         f.argtypes = [D, D, ctypes.c_long]
 
@@ -126,7 +127,7 @@ def test_log_exp_match_host_build(dev, host):
     rng = np.random.default_rng(12)
     xl = np.concatenate([10 ** rng.uniform(-307, 307, 1 << 20), rng.uniform(0.5, 2.0, 1 << 18),
                          1 + rng.uniform(-1e-6, 1e-6, 1 << 16)])
-    for fn in ("log", "log_pos"):
+    for fn in ("log", "log_pos", "log_pos_lo"):
         np.testing.assert_array_equal(dev(fn, xl), host(fn, xl), err_msg=fn)
     xe = np.concatenate([rng.uniform(-745, 709.7, 1 << 20), rng.uniform(-1, 1, 1 << 18),
                          rng.uniform(-1e-8, 1e-8, 1 << 16)])
