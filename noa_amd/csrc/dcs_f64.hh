@@ -318,7 +318,12 @@ struct F64N {
             // ~2^-92 relative of a rounding midpoint (~2^-39 of inputs); xi >=
             // 1e-66 here, so xi is positive normal
             const double xi_i = Safe ? 1. / xi : fm::recip_pos(xi);
-            const double L3e = nlog<Safe>(3. + xi_i, t), L3m = nlog<Safe>(3. + xi, t);
+            // log(3 + 1/xi) and log(3 + xi) enter D_e and D_mu, the screening
+            // logs below Phi_e and Phi_mu, all through their absolute error: on
+            // the fast path log_pos_lo (< 6.5e-16 absolute). The logs of B_e and
+            // B_mu, whose terms cancel, keep log_pos
+            const double L3e = Safe ? nlog<Safe>(3. + xi_i, t) : fm::log_pos_lo(3. + xi_i, t);
+            const double L3m = Safe ? nlog<Safe>(3. + xi, t) : fm::log_pos_lo(3. + xi, t);
             double De, Dm, Ne, Nm;
             if constexpr (Safe) {  // the reference's order (its overflow and NaN results)
                 De = 2. * (1. + 3. * beta) * L3e - rho2 - 2. * beta * (2. - rho2);
@@ -389,8 +394,8 @@ struct F64N {
                 const double Dle = fma(c.p_cLe, xe, 1.) * (ge * ge);
                 const double Dlm = c.p_15Z13 * fma(c.p_cL, xmu, qr);
                 const double r_l = fm::recip_pos(Dle * Dlm);
-                const double Le2 = fm::log_pos(c.p_AZ13_sq * (qr * qr) * xe * (Dlm * r_l), t);
-                const double Lmu = fm::log_pos(c.p_rAZ13 * qr * (Dle * r_l), t);
+                const double Le2 = fm::log_pos_lo(c.p_AZ13_sq * (qr * qr) * xe * (Dlm * r_l), t);
+                const double Lmu = fm::log_pos_lo(c.p_rAZ13 * qr * (Dle * r_l), t);
                 // one v_max each; a NaN here comes from a NaN q, which the
                 // (E - q) / q factor of the result carries anyway
                 const double Phi_mu = fmax(Bmu * Lmu, 0.);
