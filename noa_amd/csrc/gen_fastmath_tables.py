@@ -5,15 +5,21 @@ log and exp in fastmath.hh (Tang's method).
 
 All values are computed with 60-digit decimal arithmetic and rounded once.
 
-log: z in [0.6875, 1.375) is cut into 256 subintervals by the top 8 mantissa
-bits of (bits(x) - bits(0.6875)); for subinterval i with centre c_i (c = 1
-for the two subintervals that touch 1),
+log: x = 2^k z with z in [0.5, 1) (the significand with exponent field
+0x3fe); z is cut into 256 subintervals by the top 8 mantissa bits of x; for
+subinterval i with centre c_i (c = 0.5 for the first, which holds z = x / 2
+for x just above 1, and c = 1 for the last, which holds x just below 1),
   invc[i]  = 1 / c_i rounded to double,
-  logc_hi  = -log(invc[i]) rounded to a multiple of 2^-43 (so that
-             k * LN2_HI + logc_hi is exact for |k| < 2^11),
-  logc_lo  = the remainder, rounded to double.
+  logc_hi  = -log(invc[i]) rounded to a multiple of 2^-43, minus 1022 LN2_HI
+             (exact: a multiple of 2^-43 below 2^10), so that E LN2_HI +
+             logc_hi = k LN2_HI - log(invc) is exact with E = k + 1022, the
+             biased exponent field of x (no integer subtraction per call),
+  logc_lo  = the remainder of -log(invc[i]), minus 1022 LN2_LO, rounded to
+             double (exact for the first and last subintervals: LN2_LO has
+             42 significant bits, so near x = 1 the table parts cancel
+             exactly and log x keeps full relative accuracy).
   Then r = z invc - 1 lies in [-2^-9, 2^-8] (the widest ranges are those of
-  the two subintervals at 1, where r = z - 1 exactly).
+  the two subintervals at 1, where r = 2z - 1 or z - 1 exactly).
 log1p(r) = r + r^2 P(r): P is the degree-4 Chebyshev economization of the
   series (log1p(r) - r) / r^2 = sum_n (-1)^(n+1) r^n / (n + 2) over that r
   range (exact rational arithmetic: the series to degree 40 in the shifted
@@ -38,7 +44,8 @@ N_LOG = 256
 LOG_SHIFT = 44  # 52 - log2(N_LOG)
 N_EXP = 128
 LOG_DEG = 4
-OFF = 0x3fe6000000000000  # bits of 0.6875
+HALF = 0x3fe0000000000000  # bits of 0.5
+E_BIAS = 1022  # biased exponent field of x = 2^k z, z in [0.5, 1): E = k + 1022
 
 
 def d2f(d: Decimal) -> float:
@@ -59,25 +66,43 @@ def round_to_multiple(d: Decimal, q: Decimal) -> Decimal:
     return (d / q).to_integral_value() * q
 
 
+def ln2_parts():
+    """LN2_HI: ln 2 rounded to a multiple of 2^-43; LN2_LO: the remainder
+    rounded to 42 significant bits (a multiple of 2^-86), so that E LN2_LO
+    is exact for the biased exponents E < 2^11. Exact rationals."""
+    ln2 = Fraction(Decimal(2).ln())
+    hi = Fraction(round(ln2 * 2 ** 43), 2 ** 43)
+    lo = Fraction(round((ln2 - hi) * 2 ** 86), 2 ** 86)
+    assert Fraction(float(hi)) == hi and Fraction(float(lo)) == lo
+    return hi, lo
+
+
 def log_tables():
     invc, lhi, llo = [], [], []
-    q43 = Decimal(2) ** -43
+    ln2_hi, ln2_lo = ln2_parts()
     for i in range(N_LOG):
-        lo_b = OFF + (i << LOG_SHIFT)
-        hi_b = OFF + ((i + 1) << LOG_SHIFT)
-        a = Decimal(bits_to_float(lo_b))
-        b = Decimal(bits_to_float(hi_b))
+        a = Decimal(bits_to_float(HALF + (i << LOG_SHIFT)))
+        b = Decimal(bits_to_float(HALF + ((i + 1) << LOG_SHIFT))) if i + 1 < N_LOG else Decimal(1)
         c = (a + b) / 2
-        if a == 1 or b == 1:
-            # the two subintervals that touch 1 use c = 1 (r = z - 1 exactly,
-            # no cancellation between log(c) and log1p(r) as log x -> 0)
-            c = Decimal(1)
+        if i == 0:
+            c = Decimal(1) / 2  # x just above 1: r = 2 z - 1 exactly
+        elif i == N_LOG - 1:
+            c = Decimal(1)  # x just below 1: r = z - 1 exactly
         ic = d2f(Decimal(1) / c)
-        lc = -ln(Decimal(ic))
-        h = round_to_multiple(lc, q43)
+        if i == 0:
+            h, lo = -ln2_hi, -ln2_lo  # -log 2 as exactly -(LN2_HI + LN2_LO)
+        else:
+            lc = Fraction(-ln(Decimal(ic)))
+            h = Fraction(round(lc * 2 ** 43), 2 ** 43)
+            lo = lc - h
+        hf = h - E_BIAS * ln2_hi
+        assert Fraction(float(hf)) == hf  # the folded high part stays exact
+        lof = lo - E_BIAS * ln2_lo
+        if i in (0, N_LOG - 1):
+            assert Fraction(float(lof)) == lof  # exact where log x -> 0
         invc.append(ic)
-        lhi.append(d2f(h))
-        llo.append(d2f(lc - Decimal(d2f(h))))
+        lhi.append(float(hf))
+        llo.append(float(lof))
     return invc, lhi, llo
 
 
@@ -212,13 +237,12 @@ def exp_tables():
 
 def constants():
     ln2 = Decimal(2).ln()
-    ln2_hi = round_to_multiple(ln2, Decimal(2) ** -43)
-    ln2_lo = ln2 - ln2_hi
+    ln2_hi, ln2_lo = ln2_parts()
     ln2n = ln2 / N_EXP
     ln2n_hi = d2f(ln2n)
     ln2n_lo = d2f(ln2n - Decimal(ln2n_hi))
     return {
-        "LN2_HI": d2f(ln2_hi), "LN2_LO": d2f(ln2_lo),
+        "LN2_HI": float(ln2_hi), "LN2_LO": float(ln2_lo),
         "INV_LN2_N": d2f(N_EXP / ln2), "LN2_N_HI": ln2n_hi, "LN2_N_LO": ln2n_lo,
     }
 

@@ -178,3 +178,21 @@ def test_log_pos_lo_absolute_error(fm):
     big = np.abs(np.log(x)) >= 1
     assert ab[~big].max() < 6.5e-16, ab[~big].max()
     assert ulp_err(got[big], [e for e, b in zip(exact, big) if b]).max() < 1.5
+
+
+def test_log_near_one_and_powers_of_two(fm):
+    """The first and last subintervals of the table (x just above and just
+    below a power of two) take r = 2z - 1 or z - 1 exactly, and their table
+    parts cancel exactly at x = 1: full relative accuracy as log x -> 0, and
+    k ln 2 at the powers of two."""
+    getcontext().prec = 60
+    j = np.arange(1, 2001, dtype=np.float64)
+    x = np.concatenate([1 + j * 2.0 ** -52, 1 - j * 2.0 ** -53, 1 + j * 2.0 ** -30, 1 - j * 2.0 ** -31,
+                        np.ldexp(1 + j[:200] * 2.0 ** -40, 7), np.ldexp(1 - j[:200] * 2.0 ** -41, -9)])
+    err = ulp_err(fm["log"](x), [Decimal(float(v)).ln() for v in x])
+    assert err.max() < 0.8, err.max()
+    k = np.arange(-1074, 1024)
+    p2 = np.ldexp(1.0, k)
+    err = ulp_err(fm["log"](p2[k != 0]), [Decimal(int(v)) * Decimal(2).ln() for v in k[k != 0]])
+    assert err.max() < 0.8, err.max()
+    assert fm["log"](np.array([1.0]))[0] == 0.0
