@@ -130,6 +130,13 @@ def valu_roofline(n: int, kernel_ms: float, peak_tflops: float, counters: dict, 
         out["valu_instructions_per_pair"] = counters["SQ_INSTS_VALU"]
         if kind == "F64":  # FP32 instructions can issue faster than 4 cycles per wave64
             out["valu_issue_frac"] = counters["SQ_INSTS_VALU"] * n / 64 / s / VALU_WAVE_ISSUE_PER_S
+            if need[3] in counters:
+                # the FP64 transcendentals (v_rcp_f64, v_rsq_f64) run at quarter
+                # rate, four issue slots each: SQ_ACTIVE_INST_VALU equals
+                # INSTS_VALU + 3 TRANS_F64 within 0.1% on this kernel
+                slots = counters["SQ_INSTS_VALU"] + 3 * counters[need[3]]
+                out["valu_slots_per_pair"] = slots
+                out["valu_slot_frac"] = slots * n / 64 / s / VALU_WAVE_ISSUE_PER_S
     return out
 
 
