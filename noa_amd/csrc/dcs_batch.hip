@@ -72,9 +72,8 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(BatchMinWave
         const WorkItem w = items[it];
         // the element table is read-only for the whole launch: address it
         // through the constant address space so its fields stay scalar loads
-        using ConstPtr = const __attribute__((address_space(4))) DcsConsts*;
-        const DcsConsts& c = *(const DcsConsts*)((ConstPtr)(consts + w.element));
-        run_tile<P, TILE>(w.begin, w.begin + w.len, K, q, c, o, s, tab);  // ends with a barrier
+        run_tile<P, TILE>(w.begin, w.begin + w.len, K, q, (ConstsPtr)(consts + w.element), o, s,
+                          tab);  // ends with a barrier
     }
 }
 

@@ -36,7 +36,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(EvalWaves<P>
     load_fm_tables(fm_block);  // published by run_tile's first barrier
     const int64_t base = int64_t(blockIdx.x) * TILE;
     const int64_t end = (base + TILE < n) ? base + TILE : n;
-    run_tile<P, TILE>(base, end, K, q, c, o, s, fm::tab_from_block(fm_block));
+    run_tile<P, TILE>(base, end, K, q, kernarg_consts(), o, s, fm::tab_from_block(fm_block));
 }
 
 template <class P>

@@ -22,6 +22,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstddef>
 #include <cstdint>
 
 #include "dcs_consts.hh"
@@ -91,6 +92,40 @@ __device__ __forceinline__ void store_stream(T* p, T v) {
     __builtin_nontemporal_store(v, p);
 }
 
+// The launch constants, addressed in the constant address space (the kernel
+// argument segment, or the batch's element table): read with scalar loads.
+using ConstsPtr = const __attribute__((address_space(4))) DcsConsts*;
+
+// The constants for one item of a phase loop. The empty asm hides the
+// pointer's value, so the compiler re-reads the fields each item (scalar
+// loads, which cost no VALU slot) instead of keeping ~48 of them in SGPRs
+// across the whole tile: held that long, they overflow the SGPR file and
+// are spilled to VGPR lanes, and every use then costs a v_readlane (measured
+// on 2^24 pairs: 1.7% off the FP64 kernel, 5% off the mixed form, whose
+// spills were the heavier).
+__device__ __forceinline__ const DcsConsts& item_consts(ConstsPtr p) {
+    asm volatile("" : "+s"(p));
+    return *(const DcsConsts*)p;
+}
+
+// The leading arguments of a kernel that takes (K, q, n, DcsConsts c, ...):
+// the kernel-argument segment holds them in declaration order at their
+// natural alignment, as this struct lays them out.
+struct TileKernelArgs {
+    const double* K;
+    const double* q;
+    int64_t n;
+    DcsConsts c;
+};
+
+// c of such a kernel, read in place from the kernel-argument segment (the
+// address of the parameter as a C++ object is a private copy of all its
+// bytes in scratch).
+__device__ __forceinline__ ConstsPtr kernarg_consts() {
+    using KArg = const __attribute__((address_space(4))) char*;
+    return (ConstsPtr)((KArg)__builtin_amdgcn_kernarg_segment_ptr() + offsetof(TileKernelArgs, c));
+}
+
 // Copies the log/exp tables into an LDS block (visible after the next barrier).
 __device__ __forceinline__ void load_fm_tables(double* block) {
     for (int k = threadIdx.x; k < fm::TABLE_DOUBLES; k += WG) block[k] = fm::table_entry(k);
@@ -102,7 +137,7 @@ __device__ __forceinline__ void load_fm_tables(double* block) {
 // requested (no per-output branches).
 template <class P, int TILE, bool ALL>
 __device__ __forceinline__ void phase1(int64_t base, int64_t end, const double* __restrict__ K,
-                                       const double* __restrict__ q, const DcsConsts& c,
+                                       const double* __restrict__ q, ConstsPtr cp,
                                        const Outs<typename P::out_t>& o, TileLists<TILE, P::has_redo>& s,
                                        const fm::Tab& t) {
     using out_t = typename P::out_t;
@@ -118,6 +153,7 @@ __device__ __forceinline__ void phase1(int64_t base, int64_t end, const double* 
     }
 #pragma unroll 1
     for (int it = 0; it < ITEMS; it++) {
+        const DcsConsts& c = item_consts(cp);
         const int local = it * WG + tid;
         gi = base + local;
         const bool valid = gi < end;
@@ -174,7 +210,7 @@ __device__ __forceinline__ void phase1(int64_t base, int64_t end, const double* 
 // workgroup.
 template <class P, int TILE>
 __device__ __forceinline__ void run_tile(int64_t base, int64_t end, const double* __restrict__ K,
-                                         const double* __restrict__ q, const DcsConsts& c,
+                                         const double* __restrict__ q, ConstsPtr cp,
                                          const Outs<typename P::out_t>& o, TileLists<TILE, P::has_redo>& s,
                                          const fm::Tab& t) {
     using out_t = typename P::out_t;
@@ -194,12 +230,12 @@ __device__ __forceinline__ void run_tile(int64_t base, int64_t end, const double
     // spills and cost it 0.6%).
     if constexpr (!P::has_redo) {
         if (o.b && o.i && o.p && o.ph) {
-            phase1<P, TILE, true>(base, end, K, q, c, o, s, t);
+            phase1<P, TILE, true>(base, end, K, q, cp, o, s, t);
         } else {
-            phase1<P, TILE, false>(base, end, K, q, c, o, s, t);
+            phase1<P, TILE, false>(base, end, K, q, cp, o, s, t);
         }
     } else {
-        phase1<P, TILE, false>(base, end, K, q, c, o, s, t);
+        phase1<P, TILE, false>(base, end, K, q, cp, o, s, t);
     }
     __syncthreads();
 
@@ -208,6 +244,7 @@ __device__ __forceinline__ void run_tile(int64_t base, int64_t end, const double
     const uint32_t total = nph + s.n_pair;
 #pragma unroll 1
     for (uint32_t v = tid; v < total; v += WG) {
+        const DcsConsts& c = item_consts(cp);
         bool redo = false;
         uint32_t tag;
         if (v < nph) {
@@ -240,6 +277,7 @@ __device__ __forceinline__ void run_tile(int64_t base, int64_t end, const double
         const uint32_t nr = s.n_redo;
 #pragma unroll 1
         for (uint32_t v = tid; v < nr; v += WG) {
+            const DcsConsts& c = item_consts(cp);
             const uint32_t tag = s.redo[v];
             const int64_t g = base + (tag & REDO_OFFSET);
             if (tag & REDO_ALL) {

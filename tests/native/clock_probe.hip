@@ -31,7 +31,8 @@ __global__ __launch_bounds__(noa_hip::WG) __attribute__((amdgpu_waves_per_eu(4))
     noa_hip::load_fm_tables(fm_block);
     const int64_t base = int64_t(blockIdx.x) * TILE;
     const int64_t end = (base + TILE < n) ? base + TILE : n;
-    noa_hip::run_tile<noa_hip::F64, TILE>(base, end, K, q, c, o, s, noa_hip::fm::tab_from_block(fm_block));
+    noa_hip::run_tile<noa_hip::F64, TILE>(base, end, K, q, noa_hip::kernarg_consts(), o, s,
+                                          noa_hip::fm::tab_from_block(fm_block));
     if (threadIdx.x == 0) {  // run_tile ends with a barrier: the whole tile is done
         const unsigned long long c1 = clock64(), w1 = wall_clock64();
         unsigned long long* r = rec + 4 * int64_t(blockIdx.x);
