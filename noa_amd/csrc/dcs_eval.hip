@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "capi_util.hh"
 #include "dcs_consts.hh"
@@ -38,6 +39,13 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(EvalWaves<P>
     const int64_t end = (base + TILE < n) ? base + TILE : n;
     run_tile<P, TILE>(base, end, K, q, kernarg_consts(), o, s, fm::tab_from_block(fm_block));
 }
+
+// run_tile reads c in place through kernarg_consts(), which locates it by
+// TileKernelArgs: the kernel's leading parameters must stay (K, q, n, c).
+static_assert(std::is_same<decltype(&dcs_eval_kernel<F64, EVAL_TILE>),
+                           void (*)(const double*, const double*, int64_t, DcsConsts, Outs<double>)>::value,
+              "dcs_eval_kernel's parameters must begin as TileKernelArgs lays them out");
+static_assert(offsetof(TileKernelArgs, c) == 24, "three 8-byte arguments before the constants");
 
 template <class P>
 int launch_eval(const double* K, const double* q, int64_t n, const noa_dcs_element* el, double mass,

@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "../../noa_amd/csrc/dcs_tile.hh"
 
@@ -42,6 +43,12 @@ __global__ __launch_bounds__(noa_hip::WG) __attribute__((amdgpu_waves_per_eu(4))
         r[3] = w1;
     }
 }
+
+// kernarg_consts() locates c by the library's TileKernelArgs layout
+static_assert(std::is_same<decltype(&clock_kernel),
+                           void (*)(const double*, const double*, int64_t, noa_hip::DcsConsts,
+                                    noa_hip::Outs<double>, unsigned long long*)>::value,
+              "clock_kernel's parameters must begin as TileKernelArgs lays them out");
 
 }  // namespace
 
