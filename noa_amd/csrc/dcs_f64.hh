@@ -345,7 +345,7 @@ struct F64N {
                 xmu = xi1 * (1. + Nm / Dm);
             } else {  // one reciprocal for the three (D_e, D_mu > 1)
                 const double DD = De * Dm;
-                const double r_y = fm::recip_pos(DD * xi1);
+                const double r_y = Rn ? fm::recip_faithful(DD * xi1) : fm::recip_pos(DD * xi1);
                 dr = (rho21 - beta) * (DD * r_y);
                 if constexpr (Rn) {
                     const double u = xi1 * (xi1 * r_y);  // (1 + xi) / (D_e D_mu)
@@ -393,7 +393,7 @@ struct F64N {
                 const double ge = fma(c.p_cL, xe, qr);
                 const double Dle = fma(c.p_cLe, xe, 1.) * (ge * ge);
                 const double Dlm = c.p_15Z13 * fma(c.p_cL, xmu, qr);
-                const double r_l = fm::recip_pos(Dle * Dlm);
+                const double r_l = Rn ? fm::recip_faithful(Dle * Dlm) : fm::recip_pos(Dle * Dlm);
                 const double Le2 = fm::log_pos_lo(c.p_AZ13_sq * (qr * qr) * xe * (Dlm * r_l), t);
                 const double Lmu = fm::log_pos_lo(c.p_rAZ13 * qr * (Dle * r_l), t);
                 // one v_max each; a NaN here comes from a NaN q, which the
@@ -577,7 +577,7 @@ struct F64N {
                 const double S = fma(Dth, fma(lg, fma(0.5747, den, -0.3534 * q2), DR),
                                      0.635 * q2 * den * fma(12.0 * 0.015625, q2, Dth));
                 const double QQm = Q2 * Qm, gS = gPA * S;
-                const double r = fm::recip_pos(QQm * gS);
+                const double r = Rn ? fm::recip_faithful(QQm * gS) : fm::recip_pos(QQm * gS);
                 const double rq = gS * r, rg = QQm * r;
                 inv_Q2 = Qm * rq;
                 inv_Qm = Q2 * rq;

@@ -10,7 +10,7 @@
 
 namespace {
 
-enum Fn : int { FN_LOG = 0, FN_LOG_POS = 1, FN_EXP = 2, FN_EXP_FIN = 3, FN_RECIP_POS = 4, FN_RECIP_NZ = 5, FN_EXP_FIN4 = 6, FN_LOG_POS_LO = 7 };
+enum Fn : int { FN_LOG = 0, FN_LOG_POS = 1, FN_EXP = 2, FN_EXP_FIN = 3, FN_RECIP_POS = 4, FN_RECIP_NZ = 5, FN_EXP_FIN4 = 6, FN_LOG_POS_LO = 7, FN_RECIP_FAITHFUL = 8 };
 
 constexpr int WG = 256;
 
@@ -32,6 +32,7 @@ __global__ __launch_bounds__(WG) void fm_kernel(int fn, const double* __restrict
             case FN_EXP_FIN4: r = noa_hip::fm::exp_fin4(v, t); break;
             case FN_LOG_POS_LO: r = noa_hip::fm::log_pos_lo(v, t); break;
             case FN_RECIP_POS: r = noa_hip::fm::recip_pos(v); break;
+            case FN_RECIP_FAITHFUL: r = noa_hip::fm::recip_faithful(v); break;
             default: r = noa_hip::fm::recip_nz(v); break;
         }
         y[i] = r;
@@ -64,7 +65,7 @@ extern "C" int fm_dev_div(const double* a, const double* b, double* y, int64_t n
 // synchronously. Returns 0, or the hipError_t of the failing call; 1 for an
 // unknown fn or negative n.
 extern "C" int fm_dev_eval(int fn, const double* x, double* y, int64_t n) {
-    if (fn < FN_LOG || fn > FN_LOG_POS_LO || n < 0) return 1;
+    if (fn < FN_LOG || fn > FN_RECIP_FAITHFUL || n < 0) return 1;
     if (n == 0) return 0;
     const int64_t blocks = (n + WG - 1) / WG;
     const unsigned grid = unsigned(blocks < 8192 ? blocks : 8192);

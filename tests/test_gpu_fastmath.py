@@ -16,7 +16,7 @@ pytestmark = pytest.mark.gpu
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 DEV_SO = os.path.join(HERE, "native", "libfm_dev.so")
-FN = {"log": 0, "log_pos": 1, "exp": 2, "exp_fin": 3, "recip_pos": 4, "recip_nz": 5, "exp_fin4": 6, "log_pos_lo": 7}
+FN = {"log": 0, "log_pos": 1, "exp": 2, "exp_fin": 3, "recip_pos": 4, "recip_nz": 5, "exp_fin4": 6, "log_pos_lo": 7, "recip_faithful": 8}
 
 
 @pytest.fixture(scope="module")
@@ -90,6 +90,25 @@ def test_recip_pos_is_correctly_rounded(dev):
     want = 1.0 / x
     bad = np.flatnonzero(got != want)
     assert bad.size == 0, (bad.size, x[bad[:5]], got[bad[:5]], want[bad[:5]])
+
+
+def test_recip_faithful_within_one_ulp(dev):
+    """The three-instruction reciprocal: within 1 ulp of 1 / x (faithful) over
+    the whole exponent range, at powers of two and all-ones significands, and
+    equal to it on nearly every input."""
+    rng = np.random.default_rng(15)
+    x = np.concatenate([
+        10 ** rng.uniform(-307, 307, 1 << 22),
+        rng.uniform(1.0, 2.0, 1 << 20),
+        np.ldexp(1.0, np.arange(-1021, 1022)).astype(np.float64),
+        np.nextafter(np.ldexp(1.0, np.arange(-1020, 1022)), 0.0),
+        np.nextafter(np.ldexp(1.0, np.arange(-1021, 1021)), np.inf),
+    ])
+    got = dev("recip_faithful", x)
+    want = 1.0 / x
+    ulps = np.abs(got - want) / np.spacing(want)
+    assert ulps.max() <= 1.0, (ulps.max(), x[np.argmax(ulps)])
+    assert (got == want).mean() > 0.99, (got == want).mean()
 
 
 def test_recip_nz_negative_arguments(dev):
